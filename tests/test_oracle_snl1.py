@@ -1,0 +1,121 @@
+"""The CPU oracle for INSNL1/W3SNL1, pinned before it is trusted.
+
+Pins, in order of strength:
+  1. the reference's golden vectors (tests/golden/snl1_nk25_nth24.bin: Fortran INSNL1
+     tables and W3SNL1 answers for three points), bit for bit;
+  2. exact symmetries of the DIA that hold in float32 arithmetic, at other grid sizes:
+     rotation by one direction bin, mirror theta -> -theta (the two quadruplet images swap,
+     and float addition is commutative), and scaling A -> 2A (S -> 8S, D -> 4D, exact
+     for a power of two away from over/underflow);
+  3. closed forms: zero spectrum -> zero output; weights of each interpolation stencil
+     sum to one.
+A wrong table index, a swapped image or a dropped factor fails at least one of them.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from ww3gpu import synth
+
+
+def bits(x):
+    return np.ascontiguousarray(x).view(np.uint32)
+
+
+def test_tables_equal_fixture(fixture_snl1):
+    f = fixture_snl1
+    c = f.cfg
+    t = O.insnl1(f.nk, f.nth, c["xfr"], c["dth"], c["lam"], f.sig, chain=1)
+    assert list(t.dims[:5]) == list(f.dims5)
+    assert np.array_equal(bits(t.scal), bits(f.scal))
+    assert np.array_equal(t.ipim, f.ipim)
+    assert np.array_equal(t.ic, f.ic)
+    assert np.array_equal(bits(t.af11), bits(f.af11))
+
+
+def test_af11_chain_o3_differs_only_in_last_bit(fixture_snl1):
+    """-O3 WW3 computes x**11 as x^5*x^6; the fixture's -O0 build as x^3*x^8 (reference
+    docs/log/2026-10-14-snl1-bitwise-af11.md).  Both are chains of correctly rounded
+    products, so they agree to a few ulps and differ on part of the rows."""
+    f = fixture_snl1
+    c = f.cfg
+    t0 = O.insnl1(f.nk, f.nth, c["xfr"], c["dth"], c["lam"], f.sig, chain=0)
+    d = np.abs(bits(t0.af11).astype(np.int64) - bits(f.af11).astype(np.int64))
+    assert d.max() <= 4 and d.sum() > 0
+
+
+def test_w3snl1_equals_fixture(fixture_snl1):
+    f = fixture_snl1
+    c = f.cfg
+    t = O.insnl1(f.nk, f.nth, c["xfr"], c["dth"], c["lam"], f.sig, chain=1)
+    s, d = O.w3snl1(c, t, f.sig, f.a, f.cg, f.kdmean)
+    assert np.array_equal(bits(s), bits(f.s))
+    assert np.array_equal(bits(d), bits(f.d))
+
+
+def test_w3snl1_fixture_tables_give_fixture_answers(fixture_snl1):
+    f = fixture_snl1
+    c = f.cfg
+    dims = O.snl1_dims(f.nk, f.nth, c["xfr"], c["lam"])
+    ft = O.Snl1Tables(dims, f.scal, np.ascontiguousarray(f.ipim), np.ascontiguousarray(f.ic), f.af11)
+    s, d = O.w3snl1(c, ft, f.sig, f.a, f.cg, f.kdmean)
+    assert np.array_equal(bits(s), bits(f.s)) and np.array_equal(bits(d), bits(f.d))
+
+
+def test_stencil_weights_sum_to_one():
+    cfg, sig = synth.dia_config(32, 36)
+    t = O.insnl1(32, 36, cfg["xfr"], cfg["dth"], cfg["lam"], sig)
+    awg = t.scal[3:11].astype(np.float64)
+    assert abs(awg[:4].sum() - 1.0) < 1e-6 and abs(awg[4:].sum() - 1.0) < 1e-6
+    assert np.array_equal(bits(t.scal[11:]), bits((t.scal[3:11] * t.scal[3:11]).astype(np.float32)))
+
+
+@pytest.mark.parametrize("nk,nth", [(32, 24), (32, 36), (6, 8)])
+def test_zero_spectrum_gives_zero(nk, nth):
+    cfg, sig = synth.dia_config(nk, nth)
+    t = O.insnl1(nk, nth, cfg["xfr"], cfg["dth"], cfg["lam"], sig)
+    a = np.zeros((2, nk * nth), np.float32)
+    cg = np.ones((2, nk), np.float32)
+    s, d = O.w3snl1(cfg, t, sig, a, cg, np.array([3.0, 300.0], np.float32))
+    assert not s.any() and not d.any()
+
+
+def _case(nk, nth, npts=6, seed=11):
+    cfg, sig, a, cg, kd = synth.dia_workload(npts, nk, nth, seed, log_depth=(5.0, 4000.0))
+    t = O.insnl1(nk, nth, cfg["xfr"], cfg["dth"], cfg["lam"], sig)
+    return cfg, sig, t, a, cg, kd
+
+
+@pytest.mark.parametrize("nk,nth", [(32, 24), (32, 36), (25, 24), (10, 12)])
+def test_rotation_equivariance(nk, nth):
+    cfg, sig, t, a, cg, kd = _case(nk, nth)
+    s, d = O.w3snl1(cfg, t, sig, a, cg, kd)
+    rot = lambda x: np.roll(x.reshape(-1, nk, nth), 1, axis=2).reshape(x.shape)
+    s2, d2 = O.w3snl1(cfg, t, sig, rot(a), cg, kd)
+    assert np.array_equal(bits(s2), bits(rot(s))) and np.array_equal(bits(d2), bits(rot(d)))
+
+
+@pytest.mark.parametrize("nk,nth", [(32, 24), (32, 36), (25, 24), (10, 12)])
+def test_mirror_equivariance(nk, nth):
+    cfg, sig, t, a, cg, kd = _case(nk, nth)
+    s, d = O.w3snl1(cfg, t, sig, a, cg, kd)
+    idx = (-np.arange(nth)) % nth
+    mir = lambda x: x.reshape(-1, nk, nth)[:, :, idx].reshape(x.shape)
+    s2, d2 = O.w3snl1(cfg, t, sig, mir(a), cg, kd)
+    assert np.array_equal(bits(s2), bits(mir(s))) and np.array_equal(bits(d2), bits(mir(d)))
+
+
+@pytest.mark.parametrize("nk,nth", [(32, 24), (32, 36)])
+def test_cubic_scaling(nk, nth):
+    cfg, sig, t, a, cg, kd = _case(nk, nth)
+    s, d = O.w3snl1(cfg, t, sig, a, cg, kd)
+    s2, d2 = O.w3snl1(cfg, t, sig, a * np.float32(2.0), cg, kd)
+    assert np.array_equal(bits(s2), bits(s * np.float32(8.0)))
+    assert np.array_equal(bits(d2), bits(d * np.float32(4.0)))
+
+
+def test_thread_count_does_not_change_bits():
+    cfg, sig, t, a, cg, kd = _case(32, 24, npts=64)
+    s1, d1 = O.w3snl1(cfg, t, sig, a, cg, kd, nthreads=1)
+    s4, d4 = O.w3snl1(cfg, t, sig, a, cg, kd, nthreads=4)
+    assert np.array_equal(bits(s1), bits(s4)) and np.array_equal(bits(d1), bits(d4))

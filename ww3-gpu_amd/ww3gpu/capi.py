@@ -1,0 +1,85 @@
+"""ctypes declarations of every entry point in include/ww3gpu_c.h."""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_PKG), "libww3gpu.so")
+
+c_int, c_long, c_float, c_void_p, c_char_p = (ctypes.c_int, ctypes.c_long, ctypes.c_float,
+                                               ctypes.c_void_p, ctypes.c_char_p)
+f32p = ctypes.POINTER(ctypes.c_float)
+i32p = ctypes.POINTER(ctypes.c_int)
+
+
+class Status(enum.IntEnum):
+    """ww_kokkos_c.hpp:42-50 of the reference."""
+    OK = 0
+    NOT_INITIALISED = 1
+    BAD_SHAPE = 2
+    KERNEL = 3
+    RUNTIME = 4
+    UNSUPPORTED = 5
+    TABLE = 6
+
+
+class WW3GpuError(RuntimeError):
+    def __init__(self, where: str, code: int):
+        super().__init__(f"{where}: status {code} ({Status(code).name if code in Status._value2member_map_ else '?'})")
+        self.code = code
+
+
+# name -> (restype, argtypes).  Pointer arguments are declared as c_void_p so callers
+# can pass either host pointers (numpy .ctypes.data) or device pointers (tensor.data_ptr()).
+SIGNATURES = {
+    "ww_kokkos_init": (c_int, [c_int]),
+    "ww_kokkos_finalize": (None, []),
+    "ww_kokkos_flag": (c_int, [c_char_p]),
+    "ww_snl1_init": (c_int, [c_int, c_int, c_int] + [c_float] * 11 + [c_void_p]),
+    "ww_snl1": (None, [c_int, c_int] + [c_void_p] * 5),
+    "ww_snl1_enabled": (c_int, []),
+    "ww_snl1_last_error": (c_int, []),
+    "ww3gpu_snl1_dev": (c_int, [c_int, c_int] + [c_void_p] * 6),
+    "ww3gpu_snl1_tables": (c_int, [c_int] + [c_void_p] * 5),
+    "ww3gpu_snl1_set_af11": (c_int, [c_int, c_void_p, c_int]),
+    "ww3gpu_insnl1": (c_int, [c_int, c_int, c_float, c_float, c_float] + [c_void_p] * 7),
+    "ww3gpu_snl1_lds_bytes": (c_long, [c_int]),
+    "ww3gpu_devlibm_eval": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_long, c_void_p]),
+    "ww3gpu_version": (c_char_p, []),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libww3gpu.so and declare its entry points.  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libww3gpu.so not built: {path} (run __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def lib() -> ctypes.CDLL:
+    return load()
+
+
+def ptr(a) -> int:
+    """Address of a numpy array or a torch tensor."""
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def check(where: str, code: int) -> None:
+    if code != Status.OK:
+        raise WW3GpuError(where, code)
