@@ -55,10 +55,14 @@ def bench_dia(args):
     cfg, sig, a, cg, kd = synth.dia_workload(npts, nk, nth, seed=2024, log_depth=(10.0, 4000.0))
     op = Snl1(1, cfg, sig)
     nspecx = int(op.tables()["dims"][3])
+    from ww3gpu import capi
+    kernel_name = "dia_fixed" if capi.lib().ww3gpu_snl1_kernel_variant(op.igrid) == 0 else "dia_kernel"
     A, CG, KD = (torch.from_numpy(x).cuda() for x in (a, cg, kd))
     S = torch.empty_like(A)
     D = torch.empty_like(A)
-    stream = torch.cuda.current_stream()
+    # One stream for the launches and the events that time them.
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
     for _ in range(args.warmup):
         op.device(A, CG, KD, S, D, stream)
     torch.cuda.synchronize()
@@ -117,7 +121,7 @@ def bench_dia(args):
         "roofline": {
             "bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
-            "bytes_per_point": bpp, "kernel": "dia_kernel",
+            "bytes_per_point": bpp, "kernel": kernel_name,
         },
         "roofline_valu": {
             "achieved": achieved_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -136,8 +140,8 @@ def bench_dia(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="dia", choices=["dia"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()

@@ -81,7 +81,7 @@ int ww_snl1_last_error(void);
 /* ------------------------------------------------------------------------------------ */
 
 /* W3SNL1 on device-resident arrays (same shapes as ww_snl1, device pointers), enqueued
- * on `stream` (a hipStream_t; NULL = the library's stream) without synchronising.  The
+ * on `stream` (a hipStream_t; NULL = the default stream) without synchronising.  The
  * device-side counterpart of the reference's ww::snl1::snl1() on device Views
  * (kokkos/src/ww_kokkos/snl1_dia.hpp; timed by kokkos/tests/bench_snl1.cpp:93-126).
  * Returns the status (also recorded for ww_snl1_last_error). */
@@ -107,12 +107,20 @@ int ww3gpu_snl1_set_af11(int igrid, const float* af11, int n);
 int ww3gpu_insnl1(int nk, int nth, float xfr, float dth, float lam, const float* sig,
                   int* dims9, float* scal19, int* ipim, int* ic, float* af11, int* structured);
 
+/* Choose grid igrid's DIA kernel: 0 (default) uses a kernel specialised for the grid's
+ * quadruplet stencil when one exists, 1 always uses the general kernel.  Both compute the
+ * same bits.  Returns OK, NOT_INITIALISED or BAD_SHAPE. */
+int ww3gpu_snl1_set_variant(int igrid, int variant);
+
+/* The DIA kernel grid igrid's next call will run: 0 specialised, 1 general, -1 no grid. */
+int ww3gpu_snl1_kernel_variant(int igrid);
+
 /* Bytes of LDS one point of grid igrid takes in the DIA kernel, or -1. */
 long ww3gpu_snl1_lds_bytes(int igrid);
 
 /* Evaluate device math function `fn` (0 = expf) on n device floats x (x2 is the second
  * argument of two-argument functions, else unused), writing y, on `stream` (NULL = the
- * library's stream), without synchronising.  The device functions reproduce the host
+ * default stream), without synchronising.  The device functions reproduce the host
  * glibc's results; this entry lets a caller check that on any inputs.
  * Returns OK, NOT_INITIALISED, BAD_SHAPE or KERNEL. */
 int ww3gpu_devlibm_eval(int fn, const float* x, const float* x2, float* y, long n, void* stream);

@@ -84,3 +84,19 @@ def test_flag_resolution(monkeypatch):
     assert L.ww_kokkos_flag(b"WW3GPU_TEST_FLAG_ON") == 1
     assert L.ww_kokkos_flag(b"WW3GPU_TEST_FLAG_OFF") == 0
     assert L.ww_kokkos_flag(None) == 0
+
+
+def declared_arity():
+    src = open(os.path.join(ROOT, "include", "ww3gpu_c.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    out = {}
+    for name, params in re.findall(r"\b(ww_\w+|ww3gpu_\w+)\s*\(([^)]*)\)\s*;", src):
+        params = params.strip()
+        out[name] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_python_signatures_match_header_arity():
+    arity = declared_arity()
+    for name, (_, args) in capi.SIGNATURES.items():
+        assert len(args) == arity[name], f"{name}: {len(args)} ctypes args, header has {arity[name]}"

@@ -110,8 +110,11 @@ def test_cubic_scaling(nk, nth):
     cfg, sig, t, a, cg, kd = _case(nk, nth)
     s, d = O.w3snl1(cfg, t, sig, a, cg, kd)
     s2, d2 = O.w3snl1(cfg, t, sig, a * np.float32(2.0), cg, kd)
-    assert np.array_equal(bits(s2), bits(s * np.float32(8.0)))
-    assert np.array_equal(bits(d2), bits(d * np.float32(4.0)))
+    # exact for a power of two unless a product is subnormal: compare far above that range
+    ms, md = np.abs(s) > 1e-20, np.abs(d) > 1e-20
+    assert ms.mean() > 0.5 and md.mean() > 0.5
+    assert np.array_equal(bits(s2[ms]), bits((s * np.float32(8.0))[ms]))
+    assert np.array_equal(bits(d2[md]), bits((d * np.float32(4.0))[md]))
 
 
 def test_thread_count_does_not_change_bits():

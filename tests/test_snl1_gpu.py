@@ -36,9 +36,11 @@ def test_fixture_bit_exact(gpu_runtime, fixture_snl1):
     t = op.tables()
     assert np.array_equal(t["ipim"], f.ipim) and np.array_equal(t["ic"], f.ic)
     op.set_af11(f.af11)  # the fixture's AF11 (x^3*x^8 chain of its -O0 build)
-    s, d = op.host(f.a, f.cg, f.kdmean)
-    assert_bits(s, f.s, "S vs Fortran")
-    assert_bits(d, f.d, "D vs Fortran")
+    for variant in (0, 1):
+        op.set_variant(variant)
+        s, d = op.host(f.a, f.cg, f.kdmean)
+        assert_bits(s, f.s, f"variant {variant} S vs Fortran")
+        assert_bits(d, f.d, f"variant {variant} D vs Fortran")
 
 
 CASES = [
@@ -60,9 +62,14 @@ def test_matches_oracle(gpu_runtime, name, npts, nk, nth, seed, logd):
     t = O.insnl1(nk, nth, cfg["xfr"], cfg["dth"], cfg["lam"], sig)
     s_ref, d_ref = O.w3snl1(cfg, t, sig, a, cg, kd, nthreads=8)
     op = Snl1(200 + seed, cfg, sig)
-    s, d = op.host(a, cg, kd)
-    assert_bits(s, s_ref, f"{name} S")
-    assert_bits(d, d_ref, f"{name} D")
+    L = capi.lib()
+    # the NTH = 24 and 36 grids (LAMBDA 0.25, XFR 1.1) have a specialised kernel
+    assert L.ww3gpu_snl1_kernel_variant(op.igrid) == (0 if nth in (24, 36) else 1)
+    for variant in (0, 1):  # stencil-specialised kernel (where one exists), general kernel
+        op.set_variant(variant)
+        s, d = op.host(a, cg, kd)
+        assert_bits(s, s_ref, f"{name} variant {variant} S")
+        assert_bits(d, d_ref, f"{name} variant {variant} D")
 
 
 def test_device_entry_equals_host_entry(gpu_runtime):
@@ -146,6 +153,10 @@ def test_symmetries_at_full_size(gpu_runtime):
     s3, d3 = op.host(mir(a), cg, kd)
     assert_bits(s3, mir(s), "mirror S")
     assert_bits(d3, mir(d), "mirror D")
+    # A -> 2A scales every product by a power of two, exactly, unless a product falls into
+    # the subnormal range; compare where the outputs are far above it.
     s4, d4 = op.host(a * np.float32(2.0), cg, kd)
-    assert_bits(s4, s * np.float32(8.0), "scaling S")
-    assert_bits(d4, d * np.float32(4.0), "scaling D")
+    ms, md = np.abs(s) > 1e-20, np.abs(d) > 1e-20
+    assert ms.mean() > 0.5 and md.mean() > 0.5
+    assert_bits(s4[ms], (s * np.float32(8.0))[ms], "scaling S")
+    assert_bits(d4[md], (d * np.float32(4.0))[md], "scaling D")

@@ -30,7 +30,7 @@ extern "C" int ww3gpu_devlibm_eval(int fn, const float* x, const float* x2, floa
   if (!ww3gpu::runtime().ready) return WW_KOKKOS_ERR_NOT_INITIALISED;
   if (fn < 0 || fn > 0 || n < 0 || x == nullptr || y == nullptr) return WW_KOKKOS_ERR_BAD_SHAPE;
   if (n == 0) return WW_KOKKOS_OK;
-  hipStream_t st = stream != nullptr ? static_cast<hipStream_t>(stream) : ww3gpu::runtime().stream;
+  hipStream_t st = static_cast<hipStream_t>(stream);  // NULL is the default stream
   const long blocks = (n + 255) / 256;
   hipLaunchKernelGGL(ww3gpu::eval_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st,
                      fn, x, x2, y, n);

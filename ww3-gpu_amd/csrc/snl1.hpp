@@ -71,10 +71,21 @@ size_t lds_bytes(const Geom& g);
 
 /// Launch W3SNL1 over `npts` points on `stream`.  a, s, d: (NSPEC, npts) column-major
 /// device arrays; cg: (NK, npts); kdmean: (npts).  af11_row, sig: device copies of the
-/// grid's tables.  Returns hipSuccess or the launch error.
+/// grid's tables.  variant 0 picks a stencil-specialised kernel when one matches the
+/// geometry (snl1_fixed.hip), else the general one; variant 1 forces the general one.
+/// Returns hipSuccess or the launch error.
 hipError_t launch(const Geom& g, const float* af11_row, const float* sig, int npts,
                   const float* a, const float* cg, const float* kdmean, float* s, float* d,
-                  hipStream_t stream);
+                  hipStream_t stream, int variant);
+
+/// The specialised kernels: returns false when none matches `g` (nothing launched), else
+/// launches and stores the launch status in *err.
+bool launch_specialized(const Geom& g, const float* af11_row, const float* sig, int npts,
+                        const float* a, const float* cg, const float* kdmean, float* s, float* d,
+                        hipStream_t stream, hipError_t* err);
+
+/// Whether launch_specialized() has a kernel for `g`.
+bool has_specialized(const Geom& g);
 
 }  // namespace snl1
 }  // namespace ww3gpu

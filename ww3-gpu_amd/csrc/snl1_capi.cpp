@@ -22,6 +22,7 @@ struct Ctx {
   Tables t;
   DevBuf sig, af11;
   DevBuf a, cg, kd, s, d;  // staging for the host-array entry point
+  int variant = 0;         // 0: specialised kernel when available, 1: general kernel
   bool ready = false;
 };
 
@@ -66,7 +67,8 @@ int upload_tables(Ctx& c) {
 
 int run(Ctx& c, int npts, const float* a, const float* cg, const float* kd, float* s, float* d,
         hipStream_t stream) {
-  const hipError_t e = launch(c.t.g, c.af11.as<float>(), c.sig.as<float>(), npts, a, cg, kd, s, d, stream);
+  const hipError_t e =
+      launch(c.t.g, c.af11.as<float>(), c.sig.as<float>(), npts, a, cg, kd, s, d, stream, c.variant);
   if (e != hipSuccess) {
     std::fprintf(stderr, "ww3gpu: DIA launch: %s\n", hipGetErrorString(e));
     return WW_KOKKOS_ERR_KERNEL;
@@ -224,7 +226,7 @@ int ww3gpu_snl1_dev(int igrid, int npts, const float* a, const float* cg, const 
     report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww3gpu_snl1_dev: null array pointer");
     return WW_KOKKOS_ERR_BAD_SHAPE;
   }
-  hipStream_t st = stream != nullptr ? static_cast<hipStream_t>(stream) : runtime().stream;
+  hipStream_t st = static_cast<hipStream_t>(stream);  // NULL is the default stream
   const int rc = run(*c, npts, a, cg, kdmean, s, d, st);
   report(g_last, rc, "ww3gpu_snl1_dev: kernel");
   return rc;
@@ -264,6 +266,20 @@ int ww3gpu_snl1_set_af11(int igrid, const float* af11, int n) {
   if (hipMemcpy(c->af11.p, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
     return WW_KOKKOS_ERR_RUNTIME;
   return WW_KOKKOS_OK;
+}
+
+int ww3gpu_snl1_set_variant(int igrid, int variant) {
+  Ctx* c = find(igrid);
+  if (c == nullptr) return WW_KOKKOS_ERR_NOT_INITIALISED;
+  if (variant < 0 || variant > 1) return WW_KOKKOS_ERR_BAD_SHAPE;
+  c->variant = variant;
+  return WW_KOKKOS_OK;
+}
+
+int ww3gpu_snl1_kernel_variant(int igrid) {
+  Ctx* c = find(igrid);
+  if (c == nullptr) return -1;
+  return (c->variant == 0 && has_specialized(c->t.g)) ? 0 : 1;
 }
 
 long ww3gpu_snl1_lds_bytes(int igrid) {

@@ -227,12 +227,14 @@ hipError_t launch_n(const Geom& g, const float* af11_row, const float* sig, int 
 
 }  // namespace
 
-int max_bins_per_thread() { return 16; }
-
 hipError_t launch(const Geom& g, const float* af11_row, const float* sig, int npts,
                   const float* a, const float* cg, const float* kdmean, float* s, float* d,
-                  hipStream_t stream) {
+                  hipStream_t stream, int variant) {
   if (npts <= 0) return hipSuccess;
+  hipError_t err = hipSuccess;
+  if (variant == 0 &&
+      launch_specialized(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, &err))
+    return err;
   const int it = (g.nspecx + kBlock - 1) / kBlock;
   if (it <= 2) return launch_n<2>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream);
   if (it <= 4) return launch_n<4>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream);

@@ -37,7 +37,7 @@ SIGNATURES = {
     "ww_kokkos_init": (c_int, [c_int]),
     "ww_kokkos_finalize": (None, []),
     "ww_kokkos_flag": (c_int, [c_char_p]),
-    "ww_snl1_init": (c_int, [c_int, c_int, c_int] + [c_float] * 11 + [c_void_p]),
+    "ww_snl1_init": (c_int, [c_int, c_int, c_int] + [c_float] * 10 + [c_void_p]),
     "ww_snl1": (None, [c_int, c_int] + [c_void_p] * 5),
     "ww_snl1_enabled": (c_int, []),
     "ww_snl1_last_error": (c_int, []),
@@ -45,6 +45,8 @@ SIGNATURES = {
     "ww3gpu_snl1_tables": (c_int, [c_int] + [c_void_p] * 5),
     "ww3gpu_snl1_set_af11": (c_int, [c_int, c_void_p, c_int]),
     "ww3gpu_insnl1": (c_int, [c_int, c_int, c_float, c_float, c_float] + [c_void_p] * 7),
+    "ww3gpu_snl1_set_variant": (c_int, [c_int, c_int]),
+    "ww3gpu_snl1_kernel_variant": (c_int, [c_int]),
     "ww3gpu_snl1_lds_bytes": (c_long, [c_int]),
     "ww3gpu_devlibm_eval": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_long, c_void_p]),
     "ww3gpu_version": (c_char_p, []),

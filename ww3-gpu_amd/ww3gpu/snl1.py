@@ -85,6 +85,10 @@ class Snl1:
         af11 = np.ascontiguousarray(af11, np.float32)
         check("ww3gpu_snl1_set_af11", lib().ww3gpu_snl1_set_af11(self.igrid, ptr(af11), af11.size))
 
+    def set_variant(self, variant: int) -> None:
+        """0: stencil-specialised kernel when available (default); 1: general kernel."""
+        check("ww3gpu_snl1_set_variant", lib().ww3gpu_snl1_set_variant(self.igrid, variant))
+
     def lds_bytes(self) -> int:
         return int(lib().ww3gpu_snl1_lds_bytes(self.igrid))
 

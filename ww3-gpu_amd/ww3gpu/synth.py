@@ -75,7 +75,7 @@ def jonswap_points(npts: int, nk: int, nth: int, sig: np.ndarray, depth, seed: i
     sw = np.where(ff <= fp[:, None], 0.07, 0.09)
     shape = (GRAV ** 2 * (2.0 * np.pi) ** -4 * ff ** -5 * np.exp(-1.25 * (fp[:, None] / ff) ** 4) *
              gam[:, None] ** np.exp(-(ff - fp[:, None]) ** 2 / (2.0 * sw ** 2 * fp[:, None] ** 2)))
-    dirs = np.cos(0.5 * (th[None, :] - thm[:, None])) ** (2.0 * spread[:, None])
+    dirs = (np.cos(0.5 * (th[None, :] - thm[:, None])) ** 2) ** spread[:, None]
     dirs = dirs / (dirs.sum(axis=1, keepdims=True) * dth)
     # E(sigma, theta) = E(f) / (2 pi) D(theta), then scale alpha so 4 sqrt(m0) = Hs.
     e = shape[:, :, None] / (2.0 * np.pi) * dirs[:, None, :]
