@@ -77,8 +77,186 @@ int ww_snl1_enabled(void);
 int ww_snl1_last_error(void);
 
 /* ------------------------------------------------------------------------------------ */
+/* ST4: W3SPR4 with CALC_USTAR, W3SIN4, W3SDS4 and their tables                           */
+/* ------------------------------------------------------------------------------------ */
+
+/* ST4 RPAR indices, the reference's wire format (kokkos/src/ww_kokkos/st4_params.hpp:79-123).
+ * Array parameters occupy consecutive slots: SSWELLF(1:7), SSDSC(1:21), SINTAILPAR(1:5),
+ * CAPCHNK(1:10), SSDSBM(0:4). */
+enum WwSt4Rpar {
+  WW_ST4_R_DTH = 0, WW_ST4_R_XFR, WW_ST4_R_FTE, WW_ST4_R_FTF, WW_ST4_R_AAIRCMIN,
+  WW_ST4_R_AAIRGB, WW_ST4_R_IGPARS5, WW_ST4_R_WWNMEANPTAIL, WW_ST4_R_SSTXFTFTAIL,
+  WW_ST4_R_AALPHA, WW_ST4_R_BBETA, WW_ST4_R_ZZ0MAX, WW_ST4_R_ZZ0RAT, WW_ST4_R_ZZALP,
+  WW_ST4_R_SSINTHP, WW_ST4_R_TTAUWSHELTER, WW_ST4_R_SSWELLF,
+  WW_ST4_R_SSDSC = WW_ST4_R_SSWELLF + 7,
+  WW_ST4_R_SSDSBR = WW_ST4_R_SSDSC + 21, WW_ST4_R_SINTAILPAR,
+  WW_ST4_R_SSDSP = WW_ST4_R_SINTAILPAR + 5, WW_ST4_R_WWNMEANP, WW_ST4_R_SSTXFTWN,
+  WW_ST4_R_SSDSBRF1, WW_ST4_R_SSDSBINT, WW_ST4_R_SSDSBCK, WW_ST4_R_SSDSHCK, WW_ST4_R_SSDSABK,
+  WW_ST4_R_SSDSPBK, WW_ST4_R_CAPCHNK,
+  WW_ST4_R_ZZWND = WW_ST4_R_CAPCHNK + 10, WW_ST4_R_SSDSCOS, WW_ST4_R_SSDSDTH, WW_ST4_R_SSDSBT,
+  WW_ST4_R_SSDSBM,
+  WW_ST4_R_DELUST = WW_ST4_R_SSDSBM + 5, WW_ST4_R_DELALP, WW_ST4_R_DELTAUW, WW_ST4_R_DELU,
+  WW_ST4_R_DELTAIL, WW_ST4_R_DELAB,
+  WW_ST4_NRPAR
+};
+
+/* ST4 IPAR indices (st4_params.hpp:126-137). */
+enum WwSt4Ipar {
+  WW_ST4_I_NK = 0, WW_ST4_I_NTH, WW_ST4_I_NSPEC, WW_ST4_I_MK, WW_ST4_I_MTH, WW_ST4_I_MSPEC,
+  WW_ST4_I_STAB3, WW_ST4_I_IG1, WW_ST4_I_FLX5,
+  WW_ST4_NIPAR
+};
+
+/* Array identifiers of ww_st4_set_real/int (kokkos/src/ww_kokkos/st4_arrays.hpp:42-62), with
+ * the LBOUND every dimension must have: SIG(0:MK+1), SIG2(1:), DSIP(0:), DDEN(1:),
+ * DDEN2(1:), ESIN(1:), ECOS(1:), TH(1:), SATWEIGHTS(1:,1:), CUMULW(1:,1:), DCKI(1:,1:),
+ * QBI(1:,1:), SATINDICES(1:,1:) int, IKTAB(1:,1:) int, TAUT(0:,0:), TAUHFT(0:,0:),
+ * TAUHFT2(0:,0:,0:), FWTABLE(0:).  The last four are module arrays: one copy per process,
+ * the last registration wins. */
+enum WwSt4ArrayId {
+  WW_ST4_A_SIG = 0, WW_ST4_A_SIG2, WW_ST4_A_DSIP, WW_ST4_A_DDEN, WW_ST4_A_DDEN2, WW_ST4_A_ESIN,
+  WW_ST4_A_ECOS, WW_ST4_A_TH, WW_ST4_A_SATWEIGHTS, WW_ST4_A_CUMULW, WW_ST4_A_DCKI,
+  WW_ST4_A_QBI, WW_ST4_A_SATINDICES, WW_ST4_A_IKTAB, WW_ST4_A_TAUT, WW_ST4_A_TAUHFT,
+  WW_ST4_A_TAUHFT2, WW_ST4_A_FWTABLE,
+  WW_ST4_NARRAYS
+};
+
+/* Replaces ww_kokkos_c.hpp:158 (ww_st4_init).  Start (or restart) the configuration of grid
+ * igrid; nrpar/nipar must equal WW_ST4_NRPAR/WW_ST4_NIPAR.  Returns OK, NOT_INITIALISED,
+ * BAD_SHAPE, or UNSUPPORTED when ipar[WW_ST4_I_FLX5] is set. */
+int ww_st4_init(int igrid, const float* rpar, int nrpar, const int* ipar, int nipar);
+
+/* Replaces ww_kokkos_c.hpp:163-166 (ww_st4_set_real, ww_st4_set_int).  Register one array of
+ * grid igrid under WwSt4ArrayId id, copied; lbound and extent have rank entries each, as
+ * LBOUND() and SHAPE() give them.  Returns OK, NOT_INITIALISED or TABLE. */
+int ww_st4_set_real(int igrid, int id, const float* data, int rank, const int* lbound,
+                    const int* extent);
+int ww_st4_set_int(int igrid, int id, const int* data, int rank, const int* lbound,
+                   const int* extent);
+
+/* Replaces ww_kokkos_c.hpp:172 (ww_st4_commit).  Check grid igrid's arrays against its
+ * scalars (every array the reference checks, same extents) and copy them to the device.
+ * With WW_KOKKOS_ST4_TABLES=1 the saturation and cumulative tables are rebuilt from the grid
+ * arrays first (ww3gpu_insin4_tables) and one stderr line says how many values differ from
+ * the registered ones.  Returns OK, NOT_INITIALISED, TABLE, RUNTIME, or UNSUPPORTED for a
+ * configuration the kernels do not cover: SINTAILPAR(1) > 0.5 (tabulated u* and tail
+ * stress) or SDSBCHOICE = NINT(SSDSC(1)) other than 1. */
+int ww_st4_commit(int igrid);
+
+/* Replaces ww_kokkos_c.hpp:200 (ww_st4_enabled): which = 1 WW_KOKKOS_SDS4, 2 WW_KOKKOS_SIN4,
+ * 3 WW_KOKKOS_SPR4, 4 WW_KOKKOS_ST4_TABLES, each 1 when set to "1" in the environment. */
+int ww_st4_enabled(int which);
+
+/* Replaces ww_kokkos_c.hpp:203 (ww_st4_last_error). */
+int ww_st4_last_error(void);
+
+/* ------------------------------------------------------------------------------------ */
+/* W3SRCE: the source-term integration of one time step                                 */
+/* ------------------------------------------------------------------------------------ */
+
+/* W3SRCE RPAR indices (kokkos/src/ww_kokkos/srce_params.hpp:39-67). */
+enum WwSrceRpar {
+  WW_SRCE_R_DMIN = 0, WW_SRCE_R_DTMAX, WW_SRCE_R_DTMIN, WW_SRCE_R_FACTI1, WW_SRCE_R_FACTI2,
+  WW_SRCE_R_FACHFA, WW_SRCE_R_FACP, WW_SRCE_R_XFLT, WW_SRCE_R_XREL, WW_SRCE_R_FHMAX,
+  WW_SRCE_R_FFXFM, WW_SRCE_R_FFXPM, WW_SRCE_R_FFXFA, WW_SRCE_R_ICESCALES,
+  WW_SRCE_R_SLNC1 = WW_SRCE_R_ICESCALES + 4, WW_SRCE_R_FSPM, WW_SRCE_R_FSHF, WW_SRCE_R_SBTC1,
+  WW_SRCE_R_SDBC1, WW_SRCE_R_SDBC2, WW_SRCE_R_FACSD, WW_SRCE_R_FTTR, WW_SRCE_R_FLDALPHA,
+  WW_SRCE_R_TAIL_LEV, WW_SRCE_R_TAIL_TRAN1, WW_SRCE_R_TAIL_TRAN2,
+  WW_SRCE_NRPAR
+};
+
+/* W3SRCE IPAR indices (srce_params.hpp:70-84). */
+enum WwSrceIpar {
+  WW_SRCE_I_NK = 0, WW_SRCE_I_NTH, WW_SRCE_I_NSPEC, WW_SRCE_I_IQTPE, WW_SRCE_I_IICEDISP,
+  WW_SRCE_I_IC_NUMERICS, WW_SRCE_I_FDONLY, WW_SRCE_I_LAB_SWITCH, WW_SRCE_I_LN1,
+  WW_SRCE_I_SEED, WW_SRCE_I_FLD2, WW_SRCE_I_TAIL_ID,
+  WW_SRCE_NIPAR
+};
+
+/* Rows of ww_srce_batch's per-point input packet pin(WW_SRCE_NPIN, npts)
+ * (ww_kokkos_c.hpp:236-245). */
+enum WwSrceBatchIn {
+  WW_SRCE_IN_D_INP = 0, WW_SRCE_IN_U10ABS, WW_SRCE_IN_U10DIR, WW_SRCE_IN_AS, WW_SRCE_IN_ICE,
+  WW_SRCE_IN_DAIR, WW_SRCE_IN_COEF,
+  WW_SRCE_NPIN
+};
+
+/* Rows of ww_srce_batch's INOUT packet psc(WW_SRCE_NPSC, npts): W3SRCE's pointer scalars in
+ * argument order, SHAVE excepted (ww_kokkos_c.hpp:249-256). */
+enum WwSrceBatchScal {
+  WW_SRCE_SC_USTAR = 0, WW_SRCE_SC_USTDIR, WW_SRCE_SC_TAUWX, WW_SRCE_SC_TAUWY,
+  WW_SRCE_SC_DTDYN, WW_SRCE_SC_FCUT, WW_SRCE_SC_TAUOX, WW_SRCE_SC_TAUOY, WW_SRCE_SC_TAUWIX,
+  WW_SRCE_SC_TAUWIY, WW_SRCE_SC_TAUWNX, WW_SRCE_SC_TAUWNY, WW_SRCE_SC_PHIAW,
+  WW_SRCE_SC_CHARN, WW_SRCE_SC_TWS, WW_SRCE_SC_PHIOC, WW_SRCE_SC_PHIBBL, WW_SRCE_SC_PHICE,
+  WW_SRCE_SC_TAUOCX, WW_SRCE_SC_TAUOCY, WW_SRCE_SC_WNMEAN, WW_SRCE_SC_FPI,
+  WW_SRCE_NPSC
+};
+
+/* Replaces ww_kokkos_c.hpp:216-218 (ww_srce_init).  Configure grid igrid: rpar/ipar at the
+ * indices above; sig (SIG(1:nk)), dden (nk), ecos, esin, mapwn (nspec), copied.  Needs
+ * ww_snl1_init and ww_st4_commit done for the grid, with the same NK and NTH.  Returns OK,
+ * NOT_INITIALISED, BAD_SHAPE, or UNSUPPORTED when ipar[WW_SRCE_I_LAB_SWITCH] is not 1, or
+ * when SEED or FLD2 is set (source terms the kernels do not cover). */
+int ww_srce_init(int igrid, const float* rpar, int nrpar, const int* ipar, int nipar,
+                 const float* sig, const float* dden, const float* ecos, const float* esin,
+                 const int* mapwn);
+
+/* Replaces ww_kokkos_c.hpp:225-231 (ww_srce): W3SRCE for one point, the arguments in the
+ * order of the Fortran dummy list; every pointer argument is INOUT except wn1, cg1 (nk).
+ * fpi is left as it is (written only under FLD2).  Status in ww_srce_last_error(). */
+void ww_srce(int igrid, int it, int inflags24, int flogrd57, int flogrd58, float* spec,
+             const float* wn1, const float* cg1, float d_inp, float u10abs, float u10dir,
+             float as, float ice, float dtg, float dair, float coef, float* ustar,
+             float* ustdir, float* tauwx, float* tauwy, float* dtdyn, float* fcut,
+             float* tauox, float* tauoy, float* tauwix, float* tauwiy, float* tauwnx,
+             float* tauwny, float* phiaw, float* charn, float* tws, float* phioc,
+             float* whitecap, float* phibbl, float* taubbl, float* tauice, float* phice,
+             float* tauocx, float* tauocy, float* wnmean, float* fpi, int* shave);
+
+/* Replaces ww_kokkos_c.hpp:267-270 (ww_srce_batch): W3SRCE for npts points, host arrays
+ * packed point by point: spec (nspec, npts), psc (WW_SRCE_NPSC, npts), whitecap (4, npts),
+ * taubbl (2, npts), tauice (2, npts) INOUT; wn1, cg1 (nk, npts), pin (WW_SRCE_NPIN, npts)
+ * IN; shave (npts) OUT as 0/1.  Copies in, runs every point's sub-step loop to DTG, copies
+ * out, synchronises.  npts == 0 is a no-op.  IT > 0 with SINTAILPAR(4) <= 0.5 (TAUWBUG off)
+ * is UNSUPPORTED: that W3SRCE reads LLWS before setting it, and the reference hands the
+ * case to the Fortran (srce_kernel.hpp:288-290).  Status in ww_srce_last_error(). */
+void ww_srce_batch(int igrid, int it, int inflags24, int flogrd57, int flogrd58, float dtg,
+                   int npts, float* spec, const float* wn1, const float* cg1,
+                   const float* pin, float* psc, float* whitecap, float* taubbl,
+                   float* tauice, int* shave);
+
+/* Replaces ww_kokkos_c.hpp:273 (ww_srce_enabled): WW_KOKKOS_SRCE as read at init. */
+int ww_srce_enabled(void);
+
+/* Replaces ww_kokkos_c.hpp:276 (ww_srce_last_error). */
+int ww_srce_last_error(void);
+
+/* ------------------------------------------------------------------------------------ */
 /* Extensions (no counterpart in the reference ABI)                                     */
 /* ------------------------------------------------------------------------------------ */
+
+/* ww_srce_batch on device-resident arrays (same shapes, device pointers), enqueued on
+ * `stream` (NULL = the default stream).  The host waits on one small counter per sub-step
+ * (the number of points still integrating), so the call returns after the last sub-step's
+ * kernels are enqueued; the outputs are complete when `stream` is.  Returns the status
+ * (also recorded for ww_srce_last_error). */
+int ww3gpu_srce_batch_dev(int igrid, int it, int inflags24, int flogrd57, int flogrd58,
+                          float dtg, int npts, float* spec, const float* wn1, const float* cg1,
+                          const float* pin, float* psc, float* whitecap, float* taubbl,
+                          float* tauice, int* shave, void* stream);
+
+/* Sub-steps the last ww_srce_batch / ww3gpu_srce_batch_dev call of grid igrid took (the
+ * largest over its points), or -1. */
+int ww3gpu_srce_substeps(int igrid);
+
+/* INSIN4's saturation and cumulative tables on the host, from the ST4 packets and the grid
+ * arrays, as ww_st4_commit rebuilds them under WW_KOKKOS_ST4_TABLES=1
+ * (kokkos/src/ww_kokkos/st4_insin4.cpp:110-138, 239-262): satw, sati (2*(MTH/2-1)+1, MTH),
+ * cumulw (MSPEC, MSPEC), Fortran order; sig0 is SIG(0:NK+1), dsip0 DSIP(0:NK+1), ecos
+ * ECOS(1:), th TH(1:NTH).  Returns OK or BAD_SHAPE. */
+int ww3gpu_insin4_tables(const float* rpar, int nrpar, const int* ipar, int nipar,
+                         const float* sig0, const float* dsip0, const float* ecos,
+                         const float* th, float* satw, int* sati, float* cumulw);
 
 /* W3SNL1 on device-resident arrays (same shapes as ww_snl1, device pointers), enqueued
  * on `stream` (a hipStream_t; NULL = the default stream) without synchronising.  The

@@ -108,3 +108,76 @@ def expf(x: np.ndarray) -> np.ndarray:
     y = np.empty_like(x)
     lib().or_expf_many(fp(x), fp(y), x.size)
     return y
+
+
+# ----------------------------------------------------------------------------------------
+# ST4 / W3SRCE
+# ----------------------------------------------------------------------------------------
+def _vp(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _declare_srce(L):
+    if hasattr(L, "_srce_declared"):
+        return
+    L.or_insin4_tables.argtypes = [ctypes.c_void_p] * 9
+    L.or_insin4_tables.restype = None
+    L.or_srce_batch.argtypes = ([ctypes.c_void_p] * 12 + [ctypes.c_int] + [ctypes.c_void_p] * 10 +
+                                [ctypes.c_int] * 4 + [ctypes.c_float, ctypes.c_int] +
+                                [ctypes.c_void_p] * 9 + [ctypes.c_int])
+    L.or_srce_batch.restype = ctypes.c_int
+    L._srce_declared = True
+
+
+def insin4_tables(case: dict):
+    """SATWEIGHTS (nsat, NTH), SATINDICES (nsat, NTH), CUMULW (NSPEC, NSPEC), Fortran order,
+    returned as C-contiguous arrays of the transposed shapes (NTH, nsat) and (NSPEC, NSPEC)."""
+    L = lib()
+    _declare_srce(L)
+    g = case["grid"]
+    nth, nspec = g["nth"], g["nspec"]
+    nsat = 2 * (nth // 2 - 1) + 1
+    satw = np.zeros((nth, nsat), np.float32)
+    sati = np.zeros((nth, nsat), np.int32)
+    cumulw = np.zeros((nspec, nspec), np.float32)
+    L.or_insin4_tables(_vp(case["st4_rpar"]), _vp(case["st4_ipar"]), _vp(g["sig0"]), _vp(g["dsip0"]),
+                       _vp(g["ecos"]), _vp(g["th"]), _vp(satw), _vp(sati), _vp(cumulw))
+    return {"satw": satw, "sati": sati, "cumulw": cumulw, "nsat": nsat}
+
+
+def dia_tables_for(case: dict):
+    c = case["dia_cfg"]
+    return insnl1(case["nk"], case["nth"], c["xfr"], c["dth"], c["lam"], case["sig"], chain=0)
+
+
+def srce_batch(case: dict, tables: dict, dia: Snl1Tables = None, nthreads: int = 1,
+               npts: int = None):
+    """W3SRCE over the case's points (or the first npts).  Returns a dict of the INOUT arrays
+    after the call; the case is not modified."""
+    L = lib()
+    _declare_srce(L)
+    g = case["grid"]
+    n = case["npts"] if npts is None else npts
+    if dia is None:
+        dia = dia_tables_for(case)
+    c = case["dia_cfg"]
+    cfg9 = np.array([c[k] for k in ("xfr", "lam", "snlc1", "kdcon", "kdmn", "snls1", "snls2", "snls3",
+                                    "fachfe")], np.float32)
+    out = {k: np.ascontiguousarray(case[k][:n]).copy() for k in
+           ("spec", "psc", "whitecap", "taubbl", "tauice", "shave")}
+    wn1 = np.ascontiguousarray(case["wn1"][:n])
+    cg1 = np.ascontiguousarray(case["cg1"][:n])
+    pin = np.ascontiguousarray(case["pin"][:n])
+    fl = case["flags"]
+    rc = L.or_srce_batch(_vp(case["st4_rpar"]), _vp(case["st4_ipar"]), _vp(g["sig0"]), _vp(g["sig2"]),
+                         _vp(g["dsip0"]), _vp(g["dden"]), _vp(g["dden2"]), _vp(g["esin"]), _vp(g["ecos"]),
+                         _vp(g["th"]), _vp(tables["satw"]), _vp(tables["sati"]), tables["nsat"],
+                         _vp(tables["cumulw"]), None, _vp(case["srce_rpar"]), _vp(case["srce_ipar"]),
+                         _vp(g["mapwn"]), _vp(cfg9), _vp(dia.scal), _vp(dia.ipim), _vp(dia.ic),
+                         _vp(dia.af11), fl["it"], fl["inflags24"], fl["flogrd57"], fl["flogrd58"],
+                         fl["dtg"], n, _vp(out["spec"]), _vp(wn1), _vp(cg1), _vp(pin), _vp(out["psc"]),
+                         _vp(out["whitecap"]), _vp(out["taubbl"]), _vp(out["tauice"]), _vp(out["shave"]),
+                         nthreads)
+    if rc != 0:
+        raise RuntimeError("or_srce_batch: unsupported configuration or allocation failure")
+    return out

@@ -335,6 +335,23 @@ static size_t work_floats(int nth, const int* dims) {
   return (size_t)(dims[4] + nth) + 8u * (size_t)(dims[3] + nth) + (size_t)dims[0] * (size_t)nth;
 }
 
+size_t or_snl1_work_floats(int nk, int nth, float xfr, float lam) {
+  int dims[9];
+  or_snl1_dims(nk, nth, xfr, lam, dims);
+  return work_floats(nth, dims);
+}
+
+/* W3SNL1 for one point (the call W3SRCE makes), on Fortran-ordered tables. */
+void or_w3snl1_one(int nk, int nth, float xfr, float lam, float snlc1, float kdcon, float kdmn,
+                   float snls1, float snls2, float snls3, float fachfe, const float* scal,
+                   const int* ipim, const int* ic, const float* af11, const float* sig,
+                   const float* a, const float* cg, float kdmean, float* s, float* d, float* work) {
+  int dims[9];
+  or_snl1_dims(nk, nth, xfr, lam, dims);
+  w3snl1_point(nk, nth, dims, snlc1, kdcon, kdmn, snls1, snls2, snls3, fachfe, scal, ipim, ic,
+               af11, sig, a, cg, kdmean, s, d, work);
+}
+
 /*
  * W3SNL1 over `npts` points, column-major (NSPEC, npts) a/s/d and (NK, npts) cg, with the
  * tables passed in (the fixture's or or_insnl1()'s).  `nthreads` > 1 splits the points

@@ -49,6 +49,22 @@ struct Geom {
   float tpiinv;
 };
 
+/// Where a launch finds its points.  The C ABI's arrays are point-major (a point's NSPEC
+/// bins contiguous); the resident W3SRCE state is bin-major (point fastest).
+struct Layout {
+  long bin_stride;    ///< a, s, d: distance between consecutive bins of one point
+  long pt_stride;     ///< a, s, d: distance between consecutive points
+  long cg_k_stride;   ///< cg: distance between frequencies
+  long cg_pt_stride;  ///< cg: distance between points
+  const int* skip;    ///< when set, points with skip[p] != 0 are left untouched
+  int xcd_chunk;      ///< > 0: workgroup i takes point (i % 8) * xcd_chunk + i / 8
+};
+
+/// The C ABI's layout: a, s, d (NSPEC, npts) and cg (NK, npts), column-major.
+inline Layout point_major(int nk, int nspec) {
+  return Layout{1, nspec, 1, nk, nullptr, 0};
+}
+
 /// Host-side result of INSNL1 for one grid.
 struct Tables {
   Geom g{};
@@ -76,16 +92,32 @@ size_t lds_bytes(const Geom& g);
 /// Returns hipSuccess or the launch error.
 hipError_t launch(const Geom& g, const float* af11_row, const float* sig, int npts,
                   const float* a, const float* cg, const float* kdmean, float* s, float* d,
-                  hipStream_t stream, int variant);
+                  hipStream_t stream, int variant, const Layout& L);
 
 /// The specialised kernels: returns false when none matches `g` (nothing launched), else
 /// launches and stores the launch status in *err.
 bool launch_specialized(const Geom& g, const float* af11_row, const float* sig, int npts,
                         const float* a, const float* cg, const float* kdmean, float* s, float* d,
-                        hipStream_t stream, hipError_t* err);
+                        hipStream_t stream, hipError_t* err, const Layout& L);
+
+/// Workgroups a launch over `npts` points takes under `L`.
+inline unsigned grid_size(const Layout& L, int npts) {
+  return static_cast<unsigned>(L.xcd_chunk > 0 ? 8 * L.xcd_chunk : npts);
+}
 
 /// Whether launch_specialized() has a kernel for `g`.
 bool has_specialized(const Geom& g);
+
+/// Grid igrid's DIA as another module (W3SRCE) runs it: geometry, device tables, kernel
+/// choice.  Valid until the grid is re-initialised or the runtime finalised.
+struct GridView {
+  const Geom* g = nullptr;
+  const float* af11_row = nullptr;
+  const float* sig = nullptr;
+  int variant = 0;
+};
+/// False when grid igrid has no ready DIA context.
+bool grid_view(int igrid, GridView& v);
 
 }  // namespace snl1
 }  // namespace ww3gpu

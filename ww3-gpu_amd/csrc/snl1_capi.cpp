@@ -68,7 +68,8 @@ int upload_tables(Ctx& c) {
 int run(Ctx& c, int npts, const float* a, const float* cg, const float* kd, float* s, float* d,
         hipStream_t stream) {
   const hipError_t e =
-      launch(c.t.g, c.af11.as<float>(), c.sig.as<float>(), npts, a, cg, kd, s, d, stream, c.variant);
+      launch(c.t.g, c.af11.as<float>(), c.sig.as<float>(), npts, a, cg, kd, s, d, stream, c.variant,
+             point_major(c.t.g.nk, c.t.g.nspec));
   if (e != hipSuccess) {
     std::fprintf(stderr, "ww3gpu: DIA launch: %s\n", hipGetErrorString(e));
     return WW_KOKKOS_ERR_KERNEL;
@@ -77,6 +78,17 @@ int run(Ctx& c, int npts, const float* a, const float* cg, const float* kd, floa
 }
 
 }  // namespace
+
+bool grid_view(int igrid, GridView& v) {
+  Ctx* c = find(igrid);
+  if (c == nullptr) return false;
+  v.g = &c->t.g;
+  v.af11_row = c->af11.as<float>();
+  v.sig = c->sig.as<float>();
+  v.variant = c->variant;
+  return true;
+}
+
 }  // namespace snl1
 }  // namespace ww3gpu
 

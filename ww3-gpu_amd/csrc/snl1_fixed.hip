@@ -12,7 +12,7 @@
 //  * CONX and CONS*AF11 are computed once per row into LDS;
 //  * the parametric tail is continued by the threads of row NK along their own column.
 #include "devlibm.hpp"
-#include "snl1.hpp"
+#include "snl1_device.hpp"
 
 namespace ww3gpu {
 namespace snl1 {
@@ -59,10 +59,14 @@ __global__ __launch_bounds__(ST::BS) void dia_fixed(const Geom g, const float* _
                                                     const float* __restrict__ a,
                                                     const float* __restrict__ cg,
                                                     const float* __restrict__ kdmean,
-                                                    float* __restrict__ s, float* __restrict__ d) {
+                                                    float* __restrict__ s, float* __restrict__ d,
+                                                    const int npts, const Layout L) {
   constexpr int NTH = ST::NTH, H = ST::H, W = ST::W, RS = ST::RS, ZR = ST::ZR, R = ST::R,
                 BS = ST::BS;
   extern __shared__ float lds[];
+  const int p = layout_point(L, npts);
+  if (p < 0) return;  // uniform across the workgroup, before any barrier
+  const long bs = L.bin_stride, ks = L.cg_k_stride;
   const int nk = g.nk, nfrchg = g.nfrchg, nfrhgh = g.nfrhgh;
   float* const ue = lds;
   float* const sa = ue + (ZR + nfrhgh) * W;
@@ -72,9 +76,8 @@ __global__ __launch_bounds__(ST::BS) void dia_fixed(const Geom g, const float* _
   const int tid = threadIdx.x;
   const int c = tid % NTH;
   const int rr = tid / NTH;
-  const int p = blockIdx.x;
-  const float* const ap = a + static_cast<size_t>(p) * g.nspec;
-  const float* const cgp = cg + static_cast<size_t>(p) * nk;
+  const float* const ap = a + p * L.pt_stride;
+  const float* const cgp = cg + p * L.cg_pt_stride;
 
   // 1. Proportionality constant.
   const float x = fmaxo(g.kdcon * kdmean[p], g.kdmn);
@@ -84,13 +87,13 @@ __global__ __launch_bounds__(ST::BS) void dia_fixed(const Geom g, const float* _
   // Zero rows; CONX and CONS*AF11 per row.
   for (int i = tid; i < ZR * W; i += BS) ue[i] = 0.0f;
   for (int i = tid; i < ZR * RS; i += BS) sa[i] = 0.0f;
-  for (int i = tid; i < nk; i += BS) conx_row[i] = g.tpiinv / sig[i] * cgp[i];
+  for (int i = tid; i < nk; i += BS) conx_row[i] = g.tpiinv / sig[i] * cgp[i * ks];
   for (int i = tid; i < nfrchg; i += BS) caf_row[i] = cons * af11_row[i];
   __syncthreads();
 
   // 2. UE = A / CONX on rows 1..NK; the row-NK threads continue the tail up their column.
   for (int r0 = rr; r0 < nk; r0 += R) {
-    const float v = ap[r0 * NTH + c] / conx_row[r0];
+    const float v = ap[(r0 * NTH + c) * bs] / conx_row[r0];
     float* const u = ue + (r0 + ZR) * W + H + c;
     u[0] = v;
     if (c < H) u[NTH] = v;
@@ -157,8 +160,8 @@ __global__ __launch_bounds__(ST::BS) void dia_fixed(const Geom g, const float* _
   // 4. Source term and diagonal (same rows as section 3, so dcs[it] is this bin's).
   const float v0 = g.swg[0], v1 = g.swg[1], v2 = g.swg[2], v3 = g.swg[3];
   const float v4 = g.swg[4], v5 = g.swg[5], v6 = g.swg[6], v7 = g.swg[7];
-  float* const sp = s + static_cast<size_t>(p) * g.nspec;
-  float* const dp = d + static_cast<size_t>(p) * g.nspec;
+  float* const sp = s + p * L.pt_stride;
+  float* const dp = d + p * L.pt_stride;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int r0 = rr + it * R;
@@ -184,8 +187,8 @@ __global__ __launch_bounds__(ST::BS) void dia_fixed(const Geom g, const float* _
                        v5 * (q[ST::o4(5) + M1] + q[ST::o4(13) + M2]) +
                        v6 * (q[ST::o4(6) + M1] + q[ST::o4(14) + M2]) +
                        v7 * (q[ST::o4(7) + M1] + q[ST::o4(15) + M2]);
-      sp[r0 * NTH + c] = sv;
-      dp[r0 * NTH + c] = dv;
+      sp[(r0 * NTH + c) * bs] = sv;
+      dp[(r0 * NTH + c) * bs] = dv;
     }
   }
 }
@@ -206,7 +209,8 @@ size_t fixed_lds(const Geom& g) {
 
 template <class ST, int IT>
 hipError_t go(const Geom& g, const float* af11_row, const float* sig, int npts, const float* a,
-              const float* cg, const float* kdmean, float* s, float* d, hipStream_t stream) {
+              const float* cg, const float* kdmean, float* s, float* d, hipStream_t stream,
+              const Layout& L) {
   const size_t lds = fixed_lds<ST>(g);
   if (lds > 64u * 1024u) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&dia_fixed<ST, IT>),
@@ -214,22 +218,22 @@ hipError_t go(const Geom& g, const float* af11_row, const float* sig, int npts, 
                                              static_cast<int>(lds));
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((dia_fixed<ST, IT>), dim3(static_cast<unsigned>(npts)), dim3(ST::BS), lds,
-                     stream, g, af11_row, sig, a, cg, kdmean, s, d);
+  hipLaunchKernelGGL((dia_fixed<ST, IT>), dim3(grid_size(L, npts)), dim3(ST::BS), lds, stream, g,
+                     af11_row, sig, a, cg, kdmean, s, d, npts, L);
   return hipGetLastError();
 }
 
 template <class ST>
 bool try_stencil(const Geom& g, const float* af11_row, const float* sig, int npts, const float* a,
                  const float* cg, const float* kdmean, float* s, float* d, hipStream_t stream,
-                 hipError_t* err) {
+                 hipError_t* err, const Layout& L) {
   if (!matches<ST>(g) || fixed_lds<ST>(g) > 160u * 1024u) return false;
   const int it = (g.nfrchg + ST::R - 1) / ST::R;
   switch (it) {
-    case 1: *err = go<ST, 1>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream); return true;
-    case 2: *err = go<ST, 2>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream); return true;
-    case 3: *err = go<ST, 3>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream); return true;
-    case 4: *err = go<ST, 4>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream); return true;
+    case 1: *err = go<ST, 1>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, L); return true;
+    case 2: *err = go<ST, 2>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, L); return true;
+    case 3: *err = go<ST, 3>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, L); return true;
+    case 4: *err = go<ST, 4>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, L); return true;
     default: return false;
   }
 }
@@ -244,9 +248,9 @@ using St36 = Stencil<36, 1, 3, 2, -3, 12>;
 
 bool launch_specialized(const Geom& g, const float* af11_row, const float* sig, int npts,
                         const float* a, const float* cg, const float* kdmean, float* s, float* d,
-                        hipStream_t stream, hipError_t* err) {
-  return try_stencil<St24>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, err) ||
-         try_stencil<St36>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, err);
+                        hipStream_t stream, hipError_t* err, const Layout& L) {
+  return try_stencil<St24>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, err, L) ||
+         try_stencil<St36>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, err, L);
 }
 
 bool has_specialized(const Geom& g) {

@@ -19,7 +19,7 @@
 // Traffic per point: A (NSPEC) + CG (NK) + KDMEAN in, S and D (NSPEC each) out; SIG and
 // AF11 are per-grid and stay in cache.
 #include "devlibm.hpp"
-#include "snl1.hpp"
+#include "snl1_device.hpp"
 
 namespace ww3gpu {
 namespace snl1 {
@@ -36,10 +36,13 @@ __global__ __launch_bounds__(kBlock) void dia_kernel(const Geom g, const float* 
                                                      const float* __restrict__ a,
                                                      const float* __restrict__ cg,
                                                      const float* __restrict__ kdmean,
-                                                     float* __restrict__ s, float* __restrict__ d) {
+                                                     float* __restrict__ s, float* __restrict__ d,
+                                                     const int npts, const Layout L) {
   extern __shared__ float lds[];
-  const int p = blockIdx.x;
+  const int p = layout_point(L, npts);
+  if (p < 0) return;  // uniform across the workgroup, before any barrier
   const int tid = threadIdx.x;
+  const long bs = L.bin_stride, ks = L.cg_k_stride;
   const int nth = g.nth, w = g.w, h = g.h, zr = g.zr;
   const int sa_len = g.sa_rows * w;
   float* const ue = lds;
@@ -49,8 +52,8 @@ __global__ __launch_bounds__(kBlock) void dia_kernel(const Geom g, const float* 
   float* const da1m = da1p + sa_len;
   float* const da2p = da1m + sa_len;
   float* const da2m = da2p + sa_len;
-  const float* const ap = a + static_cast<size_t>(p) * g.nspec;
-  const float* const cgp = cg + static_cast<size_t>(p) * g.nk;
+  const float* const ap = a + p * L.pt_stride;
+  const float* const cgp = cg + p * L.cg_pt_stride;
 
   // Row/column of this thread's first bin and the per-iteration step.
   const int q = kBlock / nth, rem = kBlock - q * nth;
@@ -74,8 +77,8 @@ __global__ __launch_bounds__(kBlock) void dia_kernel(const Geom g, const float* 
   {
     int r0 = r_first, c = c_first;
     for (int isp = tid; isp < g.nspec; isp += kBlock) {
-      const float conx = g.tpiinv / sig[r0] * cgp[r0];
-      const float v = ap[isp] / conx;
+      const float conx = g.tpiinv / sig[r0] * cgp[r0 * ks];
+      const float v = ap[isp * bs] / conx;
       const int slot = (r0 + zr) * w + h + c;
       ue[slot] = v;
       if (c < h) ue[slot + nth] = v;
@@ -171,16 +174,16 @@ __global__ __launch_bounds__(kBlock) void dia_kernel(const Geom g, const float* 
 
   // 4. Source term and diagonal.
   {
-    float* const sp = s + static_cast<size_t>(p) * g.nspec;
-    float* const dp = d + static_cast<size_t>(p) * g.nspec;
+    float* const sp = s + p * L.pt_stride;
+    float* const dp = d + p * L.pt_stride;
     int r0 = r_first, c = c_first;
 #pragma unroll
     for (int it = 0; it < MAXIT; ++it) {
       const int isp = tid + it * kBlock;
       if (isp < g.nspec) {
         const int b = (r0 + zr) * w + h + c;
-        const float conx = g.tpiinv / sig[r0] * cgp[r0];
-        sp[isp] = conx * (-2.0f * (sa1[b] + sa2[b]) +
+        const float conx = g.tpiinv / sig[r0] * cgp[r0 * ks];
+        sp[isp * bs] = conx * (-2.0f * (sa1[b] + sa2[b]) +
                           g.awg[0] * (sa1[b + g.o4[0]] + sa2[b + g.o4[8]]) +
                           g.awg[1] * (sa1[b + g.o4[1]] + sa2[b + g.o4[9]]) +
                           g.awg[2] * (sa1[b + g.o4[2]] + sa2[b + g.o4[10]]) +
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void dia_kernel(const Geom g, const float* 
                           g.awg[5] * (sa1[b + g.o4[5]] + sa2[b + g.o4[13]]) +
                           g.awg[6] * (sa1[b + g.o4[6]] + sa2[b + g.o4[14]]) +
                           g.awg[7] * (sa1[b + g.o4[7]] + sa2[b + g.o4[15]]));
-        dp[isp] = -2.0f * dcs[it] +
+        dp[isp * bs] = -2.0f * dcs[it] +
                   g.swg[0] * (da1p[b + g.o4[0]] + da2p[b + g.o4[8]]) +
                   g.swg[1] * (da1p[b + g.o4[1]] + da2p[b + g.o4[9]]) +
                   g.swg[2] * (da1p[b + g.o4[2]] + da2p[b + g.o4[10]]) +
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void dia_kernel(const Geom g, const float* 
 template <int MAXIT>
 hipError_t launch_n(const Geom& g, const float* af11_row, const float* sig, int npts,
                     const float* a, const float* cg, const float* kdmean, float* s, float* d,
-                    hipStream_t stream) {
+                    hipStream_t stream, const Layout& L) {
   const size_t lds = lds_bytes(g);
   if (lds > 64u * 1024u) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&dia_kernel<MAXIT>),
@@ -220,8 +223,8 @@ hipError_t launch_n(const Geom& g, const float* af11_row, const float* sig, int 
                                              static_cast<int>(lds));
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(dia_kernel<MAXIT>, dim3(static_cast<unsigned>(npts)), dim3(kBlock), lds,
-                     stream, g, af11_row, sig, a, cg, kdmean, s, d);
+  hipLaunchKernelGGL(dia_kernel<MAXIT>, dim3(grid_size(L, npts)), dim3(kBlock), lds, stream, g,
+                     af11_row, sig, a, cg, kdmean, s, d, npts, L);
   return hipGetLastError();
 }
 
@@ -229,19 +232,19 @@ hipError_t launch_n(const Geom& g, const float* af11_row, const float* sig, int 
 
 hipError_t launch(const Geom& g, const float* af11_row, const float* sig, int npts,
                   const float* a, const float* cg, const float* kdmean, float* s, float* d,
-                  hipStream_t stream, int variant) {
+                  hipStream_t stream, int variant, const Layout& L) {
   if (npts <= 0) return hipSuccess;
   hipError_t err = hipSuccess;
   if (variant == 0 &&
-      launch_specialized(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, &err))
+      launch_specialized(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, &err, L))
     return err;
   const int it = (g.nspecx + kBlock - 1) / kBlock;
-  if (it <= 2) return launch_n<2>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream);
-  if (it <= 4) return launch_n<4>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream);
-  if (it <= 6) return launch_n<6>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream);
-  if (it <= 8) return launch_n<8>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream);
-  if (it <= 12) return launch_n<12>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream);
-  if (it <= 16) return launch_n<16>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream);
+  if (it <= 2) return launch_n<2>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, L);
+  if (it <= 4) return launch_n<4>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, L);
+  if (it <= 6) return launch_n<6>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, L);
+  if (it <= 8) return launch_n<8>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, L);
+  if (it <= 12) return launch_n<12>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, L);
+  if (it <= 16) return launch_n<16>(g, af11_row, sig, npts, a, cg, kdmean, s, d, stream, L);
   return hipErrorInvalidValue;
 }
 

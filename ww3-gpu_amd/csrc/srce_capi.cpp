@@ -1,0 +1,896 @@
+// ww3-gpu_amd/csrc/srce_capi.cpp
+// The ST4 and W3SRCE entry points of the C ABI: per-grid parameter packets and tables,
+// the host side of INSIN4's saturation and cumulative tables, and the host loop that
+// drives one time step's sub-steps on the device.
+//
+// Reference behaviour kept (kokkos/src/fortran_iface/st4_shim.cpp:178-209, 212-248,
+// 480-580; srce_shim.cpp:679-760, 866-930): a grid is configured by init, array
+// registrations (copied, LBOUND and type checked) and commit (every array the reference
+// checks, with the same extents); module arrays (FWTABLE and the stress tables) and DELAB
+// are one copy per process, the last registration wins; ww_srce_init needs the grid's DIA
+// and ST4 contexts with the same NK and NTH, and ww_srce_batch checks them again at every
+// call; validation failures write no output; npts == 0 is a no-op.
+//
+// The sub-step loop.  Every point of a batch integrates to DTG in its own number of
+// sub-steps.  One device sub-step is: k_step_a (W3SLN1, W3SIN4, KDMEAN), the DIA over the
+// points still integrating, k_step_c (W3SDS4, W3SDB1, W3SBT1, the time-step limit, the
+// update, W3SPR4, the tail, W3SIN4) which counts the points that are not done.  The host
+// reads that count (4 bytes) and stops at zero.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "runtime.hpp"
+#include "snl1.hpp"
+#include "srce.hpp"
+
+namespace ww3gpu {
+namespace st4h {
+namespace {
+
+// W3SRC4MD / W3GDATMD / CONSTANTS extents the commit checks (reference st4_params.hpp:39-52).
+constexpr int kSizeFwTable = 300;
+constexpr int kItauMax = 200, kJuMax = 200;
+constexpr int kIustar = 100, kIalpha = 200, kIlevTail = 50;
+constexpr int kNkhs = 2000, kNkd = 1300, kNdtab = 2000;
+
+struct Spec {
+  const char* name;
+  bool is_int;
+  int rank;
+  int lbound;
+  bool module;
+};
+
+// One row per WwSt4ArrayId: the registration contract of reference st4_arrays.hpp:64-84.
+constexpr Spec kSpecs[WW_ST4_NARRAYS] = {
+    {"SIG", false, 1, 0, false},        {"SIG2", false, 1, 1, false},
+    {"DSIP", false, 1, 0, false},       {"DDEN", false, 1, 1, false},
+    {"DDEN2", false, 1, 1, false},      {"ESIN", false, 1, 1, false},
+    {"ECOS", false, 1, 1, false},       {"TH", false, 1, 1, false},
+    {"SATWEIGHTS", false, 2, 1, false}, {"CUMULW", false, 2, 1, false},
+    {"DCKI", false, 2, 1, false},       {"QBI", false, 2, 1, false},
+    {"SATINDICES", true, 2, 1, false},  {"IKTAB", true, 2, 1, false},
+    {"TAUT", false, 2, 0, true},        {"TAUHFT", false, 2, 0, true},
+    {"TAUHFT2", false, 3, 0, true},     {"FWTABLE", false, 1, 0, true},
+};
+
+struct Arr {
+  bool set = false;
+  int rank = 0;
+  size_t ext[3] = {0, 0, 0};
+  std::vector<float> f;
+  std::vector<int> i;
+};
+
+struct Grid {
+  bool configured = false;
+  bool ready = false;
+  float r[WW_ST4_NRPAR] = {};
+  int ip[WW_ST4_NIPAR] = {};
+  Arr a[WW_ST4_NARRAYS];
+  DevBuf tables;
+  srce::St4Dev dev{};  ///< scalars and device pointers; fwtable and delab set per call
+};
+
+struct Module {
+  Arr a[WW_ST4_NARRAYS];
+  DevBuf fwtable;
+  float delab = 0.f;
+};
+
+std::map<int, Grid>& grids() {
+  static auto* m = new std::map<int, Grid>();
+  return *m;
+}
+
+Module& module() {
+  static auto* m = new Module();
+  return *m;
+}
+
+std::atomic<int> g_last{WW_KOKKOS_OK};
+bool g_hooked = false;
+
+void release_all() {
+  for (auto& kv : grids()) kv.second.tables.release();
+  grids().clear();
+  module().fwtable.release();
+}
+
+void hook() {
+  if (!g_hooked) {
+    on_finalize(release_all);
+    g_hooked = true;
+  }
+}
+
+int fnint(float x) {  // NINT of a REAL: round half away, 0 when out of range
+  const float r = std::round(x);
+  return (r > -9.2233715e18f && r < 9.2233715e18f) ? static_cast<int>(static_cast<long long>(r)) : 0;
+}
+
+template <class T>
+int set_array(int igrid, int id, const T* data, int rank, const int* lbound, const int* extent) {
+  constexpr bool is_int = std::is_same<T, int>::value;
+  const auto it = grids().find(igrid);
+  if (it == grids().end() || !it->second.configured) {
+    report(g_last, WW_KOKKOS_ERR_NOT_INITIALISED, "ww_st4_set_* before ww_st4_init for this grid");
+    return WW_KOKKOS_ERR_NOT_INITIALISED;
+  }
+  if (id < 0 || id >= WW_ST4_NARRAYS || data == nullptr || lbound == nullptr || extent == nullptr) {
+    report(g_last, WW_KOKKOS_ERR_TABLE, "ww_st4_set_*: bad array id or null pointer");
+    return WW_KOKKOS_ERR_TABLE;
+  }
+  const Spec& sp = kSpecs[id];
+  if (sp.is_int != is_int || sp.rank != rank) {
+    report(g_last, WW_KOKKOS_ERR_TABLE, (std::string("ww_st4_set_*: type or rank of ") + sp.name).c_str());
+    return WW_KOKKOS_ERR_TABLE;
+  }
+  size_t n = 1;
+  for (int r = 0; r < rank; ++r) {
+    if (lbound[r] != sp.lbound || extent[r] <= 0) {
+      report(g_last, WW_KOKKOS_ERR_TABLE,
+             (std::string("ww_st4_set_*: LBOUND or extent of ") + sp.name).c_str());
+      return WW_KOKKOS_ERR_TABLE;
+    }
+    n *= static_cast<size_t>(extent[r]);
+  }
+  Arr& dst = sp.module ? module().a[id] : it->second.a[id];
+  dst.set = true;
+  dst.rank = rank;
+  for (int r = 0; r < 3; ++r) dst.ext[r] = r < rank ? static_cast<size_t>(extent[r]) : 0;
+  if (is_int) {
+    dst.i.assign(reinterpret_cast<const int*>(data), reinterpret_cast<const int*>(data) + n);
+    dst.f.clear();
+  } else {
+    dst.f.assign(reinterpret_cast<const float*>(data), reinterpret_cast<const float*>(data) + n);
+    dst.i.clear();
+  }
+  it->second.ready = false;
+  report(g_last, WW_KOKKOS_OK, "");
+  return WW_KOKKOS_OK;
+}
+
+/// The extents the commit requires (reference st4_shim.cpp:212-248): at least the grid's
+/// for the spectral arrays, exactly the allocation's for the tables.
+bool check_arrays(const Grid& g, std::string& why) {
+  const size_t nk = g.ip[WW_ST4_I_NK], nth = g.ip[WW_ST4_I_NTH], nspec = g.ip[WW_ST4_I_NSPEC];
+  const size_t mk = g.ip[WW_ST4_I_MK], mth = g.ip[WW_ST4_I_MTH], mspec = g.ip[WW_ST4_I_MSPEC];
+  const size_t nsat = 2 * (mth / 2 - 1) + 1;
+  const Module& m = module();
+  auto need = [&](const Arr& v, int id, size_t e0, size_t e1, size_t e2, bool exact) {
+    if (!v.set) {
+      why = std::string(kSpecs[id].name) + " not registered";
+      return false;
+    }
+    const size_t e[3] = {e0, e1, e2};
+    for (int r = 0; r < v.rank; ++r)
+      if (exact ? v.ext[r] != e[r] : v.ext[r] < e[r]) {
+        why = std::string(kSpecs[id].name) + " has the wrong shape";
+        return false;
+      }
+    return true;
+  };
+  bool ok = need(g.a[WW_ST4_A_SIG], WW_ST4_A_SIG, nk + 2, 0, 0, false) &&
+            need(g.a[WW_ST4_A_SIG2], WW_ST4_A_SIG2, nspec, 0, 0, false) &&
+            need(g.a[WW_ST4_A_DSIP], WW_ST4_A_DSIP, nk + 2, 0, 0, false) &&
+            need(g.a[WW_ST4_A_DDEN], WW_ST4_A_DDEN, nk, 0, 0, false) &&
+            need(g.a[WW_ST4_A_DDEN2], WW_ST4_A_DDEN2, nspec, 0, 0, false) &&
+            need(g.a[WW_ST4_A_ESIN], WW_ST4_A_ESIN, nspec + nth, 0, 0, false) &&
+            need(g.a[WW_ST4_A_ECOS], WW_ST4_A_ECOS, nspec + nth, 0, 0, false) &&
+            need(g.a[WW_ST4_A_TH], WW_ST4_A_TH, nth, 0, 0, false) &&
+            need(g.a[WW_ST4_A_SATWEIGHTS], WW_ST4_A_SATWEIGHTS, nsat, mth, 0, true) &&
+            need(g.a[WW_ST4_A_SATINDICES], WW_ST4_A_SATINDICES, nsat, mth, 0, true) &&
+            need(g.a[WW_ST4_A_CUMULW], WW_ST4_A_CUMULW, mspec, mspec, 0, true) &&
+            need(g.a[WW_ST4_A_DCKI], WW_ST4_A_DCKI, kNkhs, kNkd, 0, true) &&
+            need(g.a[WW_ST4_A_QBI], WW_ST4_A_QBI, kNkhs, kNkd, 0, true) &&
+            need(g.a[WW_ST4_A_IKTAB], WW_ST4_A_IKTAB, mk, kNdtab, 0, true) &&
+            need(m.a[WW_ST4_A_FWTABLE], WW_ST4_A_FWTABLE, kSizeFwTable + 1, 0, 0, true);
+  if (ok && g.r[WW_ST4_R_SINTAILPAR] > 0.5f) {
+    ok = need(m.a[WW_ST4_A_TAUT], WW_ST4_A_TAUT, kItauMax + 1, kJuMax + 1, 0, true) &&
+         need(m.a[WW_ST4_A_TAUHFT], WW_ST4_A_TAUHFT, kIustar + 1, kIalpha + 1, 0, true);
+    if (ok && g.r[WW_ST4_R_TTAUWSHELTER] > 0.f)
+      ok = need(m.a[WW_ST4_A_TAUHFT2], WW_ST4_A_TAUHFT2, kIustar + 1, kIalpha + 1, kIlevTail + 1, true);
+  }
+  return ok;
+}
+
+}  // namespace
+
+/// INSIN4's SATWEIGHTS/SATINDICES and CUMULW (reference st4_insin4.cpp:110-138, 239-262),
+/// host libm: COS and REAL**REAL of the Fortran are cosf and powf, SQRT is sqrtf.
+/// satw/sati: (nsat, mth), nsat = 2*(MTH/2-1)+1, column-major; cumulw: (mspec, mspec).
+void insin4_tables(const float* r, const int* ip, const float* sig0, const float* dsip0,
+                   const float* ecos, const float* th, float* satw, int* sati, float* cumulw) {
+  const int nk = ip[WW_ST4_I_NK], nth = ip[WW_ST4_I_NTH];
+  const int mth = ip[WW_ST4_I_MTH], mspec = ip[WW_ST4_I_MSPEC];
+  const int nsat = 2 * (mth / 2 - 1) + 1;
+  const float dth = r[WW_ST4_R_DTH];
+  const float pi = static_cast<float>(3.141592653589793);
+  const float rade = 180.f / pi;
+  const size_t nsat_all = static_cast<size_t>(nsat) * mth;
+  for (size_t i = 0; i < nsat_all; ++i) sati[i] = 1;
+  if (r[WW_ST4_R_SSDSDTH] < 180.f) {
+    for (size_t i = 0; i < nsat_all; ++i) satw[i] = 0.f;
+    int sdsnth = fnint(r[WW_ST4_R_SSDSDTH] / (dth * rade));
+    if (sdsnth > nth / 2 - 1) sdsnth = nth / 2 - 1;
+    for (int ith = 1; ith <= nth; ++ith)
+      for (int ii = ith - sdsnth; ii <= ith + sdsnth; ++ii) {
+        const int jj = ii < 1 ? ii + nth : (ii > nth ? ii - nth : ii);
+        const size_t at = static_cast<size_t>(ii - (ith - sdsnth)) + static_cast<size_t>(nsat) * (ith - 1);
+        sati[at] = jj;
+        satw[at] = std::pow(std::cos(th[ith - 1] - th[jj - 1]), r[WW_ST4_R_SSDSCOS]);
+      }
+  } else {
+    for (size_t i = 0; i < nsat_all; ++i) satw[i] = 1.f;
+  }
+  std::memset(cumulw, 0, sizeof(float) * static_cast<size_t>(mspec) * mspec);
+  if (r[WW_ST4_R_SSDSC + 2] < 0.f) {
+    const float grav = 9.806f;
+    const int dik = fnint(r[WW_ST4_R_SSDSBRF1] / (r[WW_ST4_R_XFR] - 1.f));
+    for (int ik = 1; ik <= nk; ++ik) {
+      const float c = grav / sig0[ik];
+      for (int ith = 1; ith <= nth; ++ith) {
+        const size_t is = static_cast<size_t>(ith + (ik - 1) * nth);
+        for (int ik2 = 1; ik2 <= ik - dik; ++ik2) {
+          const float c2 = grav / sig0[ik2];
+          for (int ith2 = 1; ith2 <= nth; ++ith2) {
+            const size_t is2 = static_cast<size_t>(ith2 + (ik2 - 1) * nth);
+            const float cw = ecos[std::abs(ith2 - ith)];
+            cumulw[(is2 - 1) + static_cast<size_t>(mspec) * (is - 1)] =
+                std::sqrt(c * c + c2 * c2 - 2.f * c * c2 * cw) * dsip0[ik2] / (0.5f * c2) * dth;
+          }
+        }
+      }
+    }
+  }
+}
+
+namespace {
+
+/// Device copies of the grid arrays the kernels read, and the St4Dev that points at them.
+int upload(Grid& g) {
+  const int nk = g.ip[WW_ST4_I_NK], nth = g.ip[WW_ST4_I_NTH], nspec = g.ip[WW_ST4_I_NSPEC];
+  const int mspec = g.ip[WW_ST4_I_MSPEC];
+  const Arr& sw = g.a[WW_ST4_A_SATWEIGHTS];
+  const int nsat = static_cast<int>(sw.ext[0]);
+  // Layout of the block, in 4-byte words, each array 64-word aligned.
+  size_t off = 0;
+  auto take = [&](size_t n) {
+    const size_t o = off;
+    off += (n + 63) / 64 * 64;
+    return o;
+  };
+  const size_t o_sig = take(nk + 2), o_sig2 = take(nspec), o_dsip = take(nk + 2), o_dden = take(nk);
+  const size_t o_dden2 = take(nspec), o_esin = take(nspec + nth), o_ecos = take(nspec + nth);
+  const size_t o_satw = take(static_cast<size_t>(nsat) * nth);
+  const size_t o_sati = take(static_cast<size_t>(nsat) * nth);
+  const size_t o_cum = take(static_cast<size_t>(nspec) * nspec);
+  std::vector<float> h(off, 0.f);
+  auto put = [&](size_t o, const Arr& a, size_t n) { std::memcpy(&h[o], a.f.data(), n * sizeof(float)); };
+  put(o_sig, g.a[WW_ST4_A_SIG], nk + 2);
+  put(o_sig2, g.a[WW_ST4_A_SIG2], nspec);
+  put(o_dsip, g.a[WW_ST4_A_DSIP], nk + 2);
+  put(o_dden, g.a[WW_ST4_A_DDEN], nk);
+  put(o_dden2, g.a[WW_ST4_A_DDEN2], nspec);
+  put(o_esin, g.a[WW_ST4_A_ESIN], nspec + nth);
+  put(o_ecos, g.a[WW_ST4_A_ECOS], nspec + nth);
+  put(o_satw, sw, static_cast<size_t>(nsat) * nth);
+  std::memcpy(&h[o_sati], g.a[WW_ST4_A_SATINDICES].i.data(), sizeof(int) * static_cast<size_t>(nsat) * nth);
+  const std::vector<float>& cw = g.a[WW_ST4_A_CUMULW].f;
+  for (int is = 0; is < nspec; ++is)
+    std::memcpy(&h[o_cum + static_cast<size_t>(is) * nspec], &cw[static_cast<size_t>(is) * mspec],
+                sizeof(float) * nspec);
+  if (g.tables.ensure(off * sizeof(float)) != hipSuccess ||
+      hipMemcpy(g.tables.p, h.data(), off * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    return WW_KOKKOS_ERR_RUNTIME;
+  // The module's FWTABLE: one device copy, replaced at every commit.
+  Module& m = module();
+  const Arr& fw = m.a[WW_ST4_A_FWTABLE];
+  if (m.fwtable.ensure(fw.f.size() * sizeof(float)) != hipSuccess ||
+      hipMemcpy(m.fwtable.p, fw.f.data(), fw.f.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    return WW_KOKKOS_ERR_RUNTIME;
+
+  srce::St4Dev& d = g.dev;
+  const float* r = g.r;
+  d = srce::St4Dev{};
+  d.nk = nk;
+  d.nth = nth;
+  d.nspec = nspec;
+  d.stab3 = g.ip[WW_ST4_I_STAB3] != 0;
+  d.ig1 = g.ip[WW_ST4_I_IG1] != 0;
+  d.nsat = nsat;
+  d.dth = r[WW_ST4_R_DTH];
+  d.xfr = r[WW_ST4_R_XFR];
+  d.fte = r[WW_ST4_R_FTE];
+  d.ftf = r[WW_ST4_R_FTF];
+  d.aaircmin = r[WW_ST4_R_AAIRCMIN];
+  d.aairgb = r[WW_ST4_R_AAIRGB];
+  d.igpars5 = r[WW_ST4_R_IGPARS5];
+  d.wwnmeanptail = r[WW_ST4_R_WWNMEANPTAIL];
+  d.sstxftftail = r[WW_ST4_R_SSTXFTFTAIL];
+  d.aalpha = r[WW_ST4_R_AALPHA];
+  d.bbeta = r[WW_ST4_R_BBETA];
+  d.zz0max = r[WW_ST4_R_ZZ0MAX];
+  d.zz0rat = r[WW_ST4_R_ZZ0RAT];
+  d.zzalp = r[WW_ST4_R_ZZALP];
+  d.ssinthp = r[WW_ST4_R_SSINTHP];
+  d.ttauwshelter = r[WW_ST4_R_TTAUWSHELTER];
+  for (int j = 0; j < 7; ++j) d.sswellf[j] = r[WW_ST4_R_SSWELLF + j];
+  for (int j = 0; j < 21; ++j) d.ssdsc[j] = r[WW_ST4_R_SSDSC + j];
+  d.ssdsbr = r[WW_ST4_R_SSDSBR];
+  for (int j = 0; j < 5; ++j) d.sintailpar[j] = r[WW_ST4_R_SINTAILPAR + j];
+  d.ssdsp = r[WW_ST4_R_SSDSP];
+  d.wwnmeanp = r[WW_ST4_R_WWNMEANP];
+  d.sstxftwn = r[WW_ST4_R_SSTXFTWN];
+  d.ssdsbrf1 = r[WW_ST4_R_SSDSBRF1];
+  for (int j = 0; j < 10; ++j) d.capchnk[j] = r[WW_ST4_R_CAPCHNK + j];
+  d.zzwnd = r[WW_ST4_R_ZZWND];
+  d.ssdsdth = r[WW_ST4_R_SSDSDTH];
+  for (int j = 0; j < 5; ++j) d.ssdsbm[j] = r[WW_ST4_R_SSDSBM + j];
+  const float* b = g.tables.as<float>();
+  d.sig = b + o_sig;
+  d.sig2 = b + o_sig2;
+  d.dsip = b + o_dsip;
+  d.dden = b + o_dden;
+  d.dden2 = b + o_dden2;
+  d.esin = b + o_esin;
+  d.ecos = b + o_ecos;
+  d.satw = b + o_satw;
+  d.sati = reinterpret_cast<const int*>(b + o_sati);
+  d.cumulw = b + o_cum;
+  d.fwtable = nullptr;
+  return WW_KOKKOS_OK;
+}
+
+/// The configurations the kernels translate (others are refused at commit).
+bool supported(const Grid& g, std::string& why) {
+  if (g.r[WW_ST4_R_SINTAILPAR] > 0.5f) {
+    why = "SINTAILPAR(1) > 0.5 (tabulated u* and tail stress) is not supported";
+    return false;
+  }
+  if (fnint(g.r[WW_ST4_R_SSDSC]) != 1) {
+    why = "SDSBCHOICE = NINT(SSDSC(1)) other than 1 is not supported";
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+/// Grid igrid's committed ST4 state for W3SRCE, with the module's FWTABLE and DELAB as of
+/// this call.  False when the grid is not committed.
+bool grid_state(int igrid, srce::St4Dev& out) {
+  if (!runtime().ready) return false;
+  const auto it = grids().find(igrid);
+  if (it == grids().end() || !it->second.ready) return false;
+  out = it->second.dev;
+  out.fwtable = module().fwtable.as<float>();
+  out.delab = module().delab;
+  return true;
+}
+
+int last_status() { return g_last.load(std::memory_order_relaxed); }
+
+}  // namespace st4h
+
+// ======================================================================================
+// W3SRCE
+// ======================================================================================
+namespace srceh {
+namespace {
+
+using srce::SrceDev;
+using srce::St4Dev;
+using srce::State;
+using srce::StepFlags;
+
+struct Grid {
+  bool ready = false;
+  float r[WW_SRCE_NRPAR] = {};
+  int ip[WW_SRCE_NIPAR] = {};
+  SrceDev dev{};
+  DevBuf tables;  ///< sig, dden, ecos, esin, mapwn
+  DevBuf state;   ///< the resident state of the last batch
+  DevBuf stage;   ///< device copies of a host-array call's arrays
+  int* counter = nullptr;  ///< pinned: the host's copy of one sub-step's active count
+  int substeps = -1;
+};
+
+std::map<int, Grid>& grids() {
+  static auto* m = new std::map<int, Grid>();
+  return *m;
+}
+
+std::atomic<int> g_last{WW_KOKKOS_OK};
+bool g_hooked = false;
+
+void release_grid(Grid& g) {
+  g.tables.release();
+  g.state.release();
+  g.stage.release();
+  if (g.counter != nullptr) (void)hipHostFree(g.counter);
+  g.counter = nullptr;
+}
+
+void release_all() {
+  for (auto& kv : grids()) release_grid(kv.second);
+  grids().clear();
+}
+
+Grid* find(int igrid) {
+  if (!runtime().ready) return nullptr;
+  const auto it = grids().find(igrid);
+  if (it == grids().end() || !it->second.ready) return nullptr;
+  return &it->second;
+}
+
+size_t al64(size_t n) { return (n + 63) / 64 * 64; }
+
+/// The context a compute call needs: W3SRCE grid, ST4 state, DIA view, consistent shapes.
+int resolve(int igrid, const char* who, Grid*& g, St4Dev& q, snl1::GridView& dv) {
+  g = find(igrid);
+  if (g == nullptr) {
+    report(g_last, WW_KOKKOS_ERR_NOT_INITIALISED, (std::string(who) + " before ww_srce_init for this grid").c_str());
+    return WW_KOKKOS_ERR_NOT_INITIALISED;
+  }
+  if (!st4h::grid_state(igrid, q) || !snl1::grid_view(igrid, dv)) {
+    report(g_last, WW_KOKKOS_ERR_NOT_INITIALISED,
+           (std::string(who) + ": the DIA or ST4 context of this grid is gone").c_str());
+    return WW_KOKKOS_ERR_NOT_INITIALISED;
+  }
+  if (dv.g->nk != g->dev.nk || dv.g->nth != g->dev.nth || q.nk != g->dev.nk || q.nth != g->dev.nth) {
+    report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, (std::string(who) + ": NK/NTH differ from the DIA or ST4 context").c_str());
+    return WW_KOKKOS_ERR_BAD_SHAPE;
+  }
+  return WW_KOKKOS_OK;
+}
+
+/// One call's sub-step loop on device arrays (point-major, the C ABI's shapes).
+int run(Grid& g, const St4Dev& q, const snl1::GridView& dv, const StepFlags& f, int npts, float* spec,
+        const float* wn1, const float* cg1, const float* pin, float* psc, float* whitecap,
+        float* taubbl, float* tauice, int* shave, hipStream_t st, const char* who) {
+  const int nk = g.dev.nk, nth = g.dev.nth;
+  if (g.state.ensure(srce::state_bytes(nk, nth, npts)) != hipSuccess) {
+    report(g_last, WW_KOKKOS_ERR_RUNTIME, (std::string(who) + ": device allocation").c_str());
+    return WW_KOKKOS_ERR_RUNTIME;
+  }
+  if (g.counter == nullptr && hipHostMalloc(reinterpret_cast<void**>(&g.counter), sizeof(int)) != hipSuccess) {
+    g.counter = nullptr;
+    report(g_last, WW_KOKKOS_ERR_RUNTIME, (std::string(who) + ": pinned allocation").c_str());
+    return WW_KOKKOS_ERR_RUNTIME;
+  }
+  const State s = srce::carve_state(g.state.p, nk, nth, npts);
+  auto fail = [&](hipError_t e, const char* what) {
+    std::string m = std::string(who) + ": " + what + ": " + hipGetErrorString(e);
+    report(g_last, WW_KOKKOS_ERR_KERNEL, m.c_str());
+    return WW_KOKKOS_ERR_KERNEL;
+  };
+  hipError_t e = srce::k_pack(s, nk, nth, spec, wn1, cg1, pin, psc, whitecap, taubbl, tauice, st);
+  if (e != hipSuccess) return fail(e, "pack");
+  e = srce::k_init(s, q, g.dev, f, st);
+  if (e != hipSuccess) return fail(e, "init");
+  // The DIA reads and writes the bin-major state in place; points already done are skipped.
+  // Workgroups are dealt to the 8 XCDs in contiguous runs of points, so the cache lines of
+  // one bin (64 consecutive points) stay in one XCD's L2.
+  const int chunk = (npts + 7) / 8;
+  const snl1::Layout L{s.P, 1, s.P, 1, s.isc + static_cast<size_t>(srce::I_DONE) * s.P, chunk};
+  int iter = 0;
+  for (;; ++iter) {
+    if (iter == srce::kMaxIter) {
+      report(g_last, WW_KOKKOS_ERR_KERNEL,
+             (std::string(who) + ": the sub-step loop did not reach DTG within its step limit").c_str());
+      return WW_KOKKOS_ERR_KERNEL;
+    }
+    e = srce::k_step_a(s, q, g.dev, f, iter, st);
+    if (e != hipSuccess) return fail(e, "sub-step (input terms)");
+    e = snl1::launch(*dv.g, dv.af11_row, dv.sig, npts, s.spec, s.cg1, s.kdmean, s.vsnl, s.vdnl, st,
+                     dv.variant, L);
+    if (e != hipSuccess) return fail(e, "sub-step (DIA)");
+    e = srce::k_step_c(s, q, g.dev, f, iter, st);
+    if (e != hipSuccess) return fail(e, "sub-step (dissipation and update)");
+    e = hipMemcpyAsync(g.counter, s.active + iter, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(e, "sub-step count");
+    if (*g.counter == 0) break;
+  }
+  g.substeps = iter + 1;
+  e = srce::k_finish(s, q, g.dev, f, st);
+  if (e != hipSuccess) return fail(e, "finish");
+  e = srce::k_unpack(s, nk, nth, spec, psc, whitecap, taubbl, tauice, shave, st);
+  if (e != hipSuccess) return fail(e, "unpack");
+  return WW_KOKKOS_OK;
+}
+
+bool args_ok(int npts, const void* spec, const void* wn1, const void* cg1, const void* pin, const void* psc,
+             const void* whitecap, const void* taubbl, const void* tauice, const void* shave) {
+  return npts >= 0 && (npts == 0 || (spec != nullptr && wn1 != nullptr && cg1 != nullptr && pin != nullptr &&
+                                     psc != nullptr && whitecap != nullptr && taubbl != nullptr &&
+                                     tauice != nullptr && shave != nullptr));
+}
+
+int check_step(const St4Dev& q, int it, const char* who) {
+  if (it != 0 && !(q.sintailpar[3] > 0.5f)) {
+    report(g_last, WW_KOKKOS_ERR_UNSUPPORTED,
+           (std::string(who) + ": IT > 0 with TAUWBUG off reads LLWS before it is set").c_str());
+    return WW_KOKKOS_ERR_UNSUPPORTED;
+  }
+  return WW_KOKKOS_OK;
+}
+
+int batch_host(int igrid, const StepFlags& f, int npts, float* spec, const float* wn1, const float* cg1,
+               const float* pin, float* psc, float* whitecap, float* taubbl, float* tauice, int* shave,
+               const char* who) {
+  Grid* g = nullptr;
+  St4Dev q;
+  snl1::GridView dv;
+  int rc = resolve(igrid, who, g, q, dv);
+  if (rc != WW_KOKKOS_OK) return rc;
+  if (!args_ok(npts, spec, wn1, cg1, pin, psc, whitecap, taubbl, tauice, shave)) {
+    report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, (std::string(who) + ": npts < 0 or a null array").c_str());
+    return WW_KOKKOS_ERR_BAD_SHAPE;
+  }
+  if ((rc = check_step(q, f.it, who)) != WW_KOKKOS_OK) return rc;
+  if (npts == 0) {
+    report(g_last, WW_KOKKOS_OK, "");
+    return WW_KOKKOS_OK;
+  }
+  const size_t n = static_cast<size_t>(npts);
+  const size_t nspec = g->dev.nspec, nk = g->dev.nk;
+  const size_t w_spec = al64(nspec * n), w_k = al64(nk * n), w_pin = al64(WW_SRCE_NPIN * n);
+  const size_t w_psc = al64(WW_SRCE_NPSC * n), w_4 = al64(4 * n), w_2 = al64(2 * n), w_1 = al64(n);
+  const size_t words = w_spec + 2 * w_k + w_pin + w_psc + w_4 + 2 * w_2 + w_1;
+  if (g->stage.ensure(words * sizeof(float)) != hipSuccess) {
+    report(g_last, WW_KOKKOS_ERR_RUNTIME, (std::string(who) + ": device allocation").c_str());
+    return WW_KOKKOS_ERR_RUNTIME;
+  }
+  float* b = g->stage.as<float>();
+  float* d_spec = b;
+  float* d_wn1 = d_spec + w_spec;
+  float* d_cg1 = d_wn1 + w_k;
+  float* d_pin = d_cg1 + w_k;
+  float* d_psc = d_pin + w_pin;
+  float* d_wc = d_psc + w_psc;
+  float* d_tb = d_wc + w_4;
+  float* d_ti = d_tb + w_2;
+  int* d_sh = reinterpret_cast<int*>(d_ti + w_2);
+  hipStream_t st = runtime().stream;
+  const auto h2d = [&](void* d, const void* h, size_t bytes) {
+    return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+  };
+  if (!h2d(d_spec, spec, nspec * n * 4) || !h2d(d_wn1, wn1, nk * n * 4) || !h2d(d_cg1, cg1, nk * n * 4) ||
+      !h2d(d_pin, pin, WW_SRCE_NPIN * n * 4) || !h2d(d_psc, psc, WW_SRCE_NPSC * n * 4) ||
+      !h2d(d_wc, whitecap, 4 * n * 4) || !h2d(d_tb, taubbl, 2 * n * 4) || !h2d(d_ti, tauice, 2 * n * 4)) {
+    report(g_last, WW_KOKKOS_ERR_RUNTIME, (std::string(who) + ": host-to-device copy").c_str());
+    return WW_KOKKOS_ERR_RUNTIME;
+  }
+  rc = run(*g, q, dv, f, npts, d_spec, d_wn1, d_cg1, d_pin, d_psc, d_wc, d_tb, d_ti, d_sh, st, who);
+  if (rc != WW_KOKKOS_OK) {
+    (void)hipStreamSynchronize(st);
+    return rc;
+  }
+  const auto d2h = [&](void* h, const void* d, size_t bytes) {
+    return hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
+  };
+  if (!d2h(spec, d_spec, nspec * n * 4) || !d2h(psc, d_psc, WW_SRCE_NPSC * n * 4) ||
+      !d2h(whitecap, d_wc, 4 * n * 4) || !d2h(taubbl, d_tb, 2 * n * 4) || !d2h(tauice, d_ti, 2 * n * 4) ||
+      !d2h(shave, d_sh, n * 4) || hipStreamSynchronize(st) != hipSuccess) {
+    report(g_last, WW_KOKKOS_ERR_RUNTIME, (std::string(who) + ": device-to-host copy").c_str());
+    return WW_KOKKOS_ERR_RUNTIME;
+  }
+  report(g_last, WW_KOKKOS_OK, "");
+  return WW_KOKKOS_OK;
+}
+
+}  // namespace
+}  // namespace srceh
+}  // namespace ww3gpu
+
+using namespace ww3gpu;
+
+extern "C" {
+
+// ------------------------------------------------------------------------------------
+// ST4
+// ------------------------------------------------------------------------------------
+int ww_st4_init(int igrid, const float* rpar, int nrpar, const int* ipar, int nipar) {
+  using namespace ww3gpu::st4h;
+  grids().erase(igrid);
+  if (!runtime().ready) {
+    report(g_last, WW_KOKKOS_ERR_NOT_INITIALISED, "ww_st4_init before ww_kokkos_init");
+    return WW_KOKKOS_ERR_NOT_INITIALISED;
+  }
+  hook();
+  if (nrpar != WW_ST4_NRPAR || nipar != WW_ST4_NIPAR || rpar == nullptr || ipar == nullptr) {
+    report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww_st4_init: RPAR/IPAR length differs from the ST4 packet");
+    return WW_KOKKOS_ERR_BAD_SHAPE;
+  }
+  if (ipar[WW_ST4_I_FLX5] != 0) {
+    report(g_last, WW_KOKKOS_ERR_UNSUPPORTED, "ww_st4_init: W3_FLX5 is not supported");
+    return WW_KOKKOS_ERR_UNSUPPORTED;
+  }
+  const int nk = ipar[WW_ST4_I_NK], nth = ipar[WW_ST4_I_NTH], nspec = ipar[WW_ST4_I_NSPEC];
+  if (nk <= 0 || nth <= 0 || nspec != nk * nth || ipar[WW_ST4_I_MK] < nk || ipar[WW_ST4_I_MTH] < nth ||
+      ipar[WW_ST4_I_MSPEC] < nspec) {
+    report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww_st4_init: NK, NTH, NSPEC, MK, MTH, MSPEC inconsistent");
+    return WW_KOKKOS_ERR_BAD_SHAPE;
+  }
+  Grid& g = grids()[igrid];
+  std::memcpy(g.r, rpar, sizeof g.r);
+  std::memcpy(g.ip, ipar, sizeof g.ip);
+  g.configured = true;
+  module().delab = rpar[WW_ST4_R_DELAB];  // module scalar: the last grid to initialise wins
+  report(g_last, WW_KOKKOS_OK, "");
+  return WW_KOKKOS_OK;
+}
+
+int ww_st4_set_real(int igrid, int id, const float* data, int rank, const int* lbound, const int* extent) {
+  return ww3gpu::st4h::set_array<float>(igrid, id, data, rank, lbound, extent);
+}
+
+int ww_st4_set_int(int igrid, int id, const int* data, int rank, const int* lbound, const int* extent) {
+  return ww3gpu::st4h::set_array<int>(igrid, id, data, rank, lbound, extent);
+}
+
+int ww_st4_commit(int igrid) {
+  using namespace ww3gpu::st4h;
+  const auto it = grids().find(igrid);
+  if (it == grids().end() || !it->second.configured || !runtime().ready) {
+    report(g_last, WW_KOKKOS_ERR_NOT_INITIALISED, "ww_st4_commit before ww_st4_init for this grid");
+    return WW_KOKKOS_ERR_NOT_INITIALISED;
+  }
+  Grid& g = it->second;
+  std::string why;
+  if (!check_arrays(g, why)) {
+    report(g_last, WW_KOKKOS_ERR_TABLE, ("ww_st4_commit: " + why).c_str());
+    return WW_KOKKOS_ERR_TABLE;
+  }
+  if (!supported(g, why)) {
+    report(g_last, WW_KOKKOS_ERR_UNSUPPORTED, ("ww_st4_commit: " + why).c_str());
+    return WW_KOKKOS_ERR_UNSUPPORTED;
+  }
+  // The saturation integral gathers A(IS0 + SATINDICES(J,ITH)): every entry must name a
+  // direction of the grid, or the kernel would read outside the point's spectrum.
+  for (const int v : g.a[WW_ST4_A_SATINDICES].i)
+    if (v < 1 || v > g.ip[WW_ST4_I_NTH]) {
+      report(g_last, WW_KOKKOS_ERR_TABLE, "ww_st4_commit: SATINDICES outside 1..NTH");
+      return WW_KOKKOS_ERR_TABLE;
+    }
+  if (ww_kokkos_flag("WW_KOKKOS_ST4_TABLES")) {
+    Arr& sw = g.a[WW_ST4_A_SATWEIGHTS];
+    Arr& si = g.a[WW_ST4_A_SATINDICES];
+    Arr& cw = g.a[WW_ST4_A_CUMULW];
+    std::vector<float> w2(sw.f.size()), c2(cw.f.size());
+    std::vector<int> i2(si.i.size());
+    insin4_tables(g.r, g.ip, g.a[WW_ST4_A_SIG].f.data(), g.a[WW_ST4_A_DSIP].f.data(),
+                  g.a[WW_ST4_A_ECOS].f.data(), g.a[WW_ST4_A_TH].f.data(), w2.data(), i2.data(), c2.data());
+    long long diff = 0;
+    for (size_t i = 0; i < w2.size(); ++i) diff += std::memcmp(&w2[i], &sw.f[i], 4) != 0;
+    for (size_t i = 0; i < i2.size(); ++i) diff += i2[i] != si.i[i];
+    for (size_t i = 0; i < c2.size(); ++i) diff += std::memcmp(&c2[i], &cw.f[i], 4) != 0;
+    std::fprintf(stderr, "ww3gpu: ww_st4_commit grid %d: INSIN4 tables rebuilt, %lld values differ\n", igrid,
+                 diff);
+    sw.f.swap(w2);
+    si.i.swap(i2);
+    cw.f.swap(c2);
+  }
+  const int rc = upload(g);
+  if (rc != WW_KOKKOS_OK) {
+    report(g_last, rc, "ww_st4_commit: device copy");
+    return rc;
+  }
+  g.ready = true;
+  report(g_last, WW_KOKKOS_OK, "");
+  return WW_KOKKOS_OK;
+}
+
+int ww_st4_enabled(int which) {
+  switch (which) {
+    case 1: return ww_kokkos_flag("WW_KOKKOS_SDS4");
+    case 2: return ww_kokkos_flag("WW_KOKKOS_SIN4");
+    case 3: return ww_kokkos_flag("WW_KOKKOS_SPR4");
+    case 4: return ww_kokkos_flag("WW_KOKKOS_ST4_TABLES");
+    default: return 0;
+  }
+}
+
+int ww_st4_last_error(void) { return ww3gpu::st4h::last_status(); }
+
+int ww3gpu_insin4_tables(const float* rpar, int nrpar, const int* ipar, int nipar, const float* sig0,
+                         const float* dsip0, const float* ecos, const float* th, float* satw, int* sati,
+                         float* cumulw) {
+  if (nrpar != WW_ST4_NRPAR || nipar != WW_ST4_NIPAR || rpar == nullptr || ipar == nullptr || sig0 == nullptr ||
+      dsip0 == nullptr || ecos == nullptr || th == nullptr || satw == nullptr || sati == nullptr ||
+      cumulw == nullptr)
+    return WW_KOKKOS_ERR_BAD_SHAPE;
+  const int nk = ipar[WW_ST4_I_NK], nth = ipar[WW_ST4_I_NTH];
+  if (nk <= 0 || nth <= 0 || ipar[WW_ST4_I_MTH] < nth || ipar[WW_ST4_I_MSPEC] < nk * nth)
+    return WW_KOKKOS_ERR_BAD_SHAPE;
+  ww3gpu::st4h::insin4_tables(rpar, ipar, sig0, dsip0, ecos, th, satw, sati, cumulw);
+  return WW_KOKKOS_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// W3SRCE
+// ------------------------------------------------------------------------------------
+int ww_srce_init(int igrid, const float* rpar, int nrpar, const int* ipar, int nipar, const float* sig,
+                 const float* dden, const float* ecos, const float* esin, const int* mapwn) {
+  using namespace ww3gpu::srceh;
+  if (grids().count(igrid)) {
+    release_grid(grids()[igrid]);
+    grids().erase(igrid);
+  }
+  if (!runtime().ready) {
+    report(g_last, WW_KOKKOS_ERR_NOT_INITIALISED, "ww_srce_init before ww_kokkos_init");
+    return WW_KOKKOS_ERR_NOT_INITIALISED;
+  }
+  if (!g_hooked) {
+    on_finalize(release_all);
+    g_hooked = true;
+  }
+  if (nrpar != WW_SRCE_NRPAR || nipar != WW_SRCE_NIPAR || rpar == nullptr || ipar == nullptr || sig == nullptr ||
+      dden == nullptr || ecos == nullptr || esin == nullptr || mapwn == nullptr) {
+    report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww_srce_init: RPAR/IPAR length differs from the W3SRCE packet");
+    return WW_KOKKOS_ERR_BAD_SHAPE;
+  }
+  if (ipar[WW_SRCE_I_LAB_SWITCH] != 1) {
+    report(g_last, WW_KOKKOS_ERR_UNSUPPORTED, "ww_srce_init: only the lab switch's source terms are translated");
+    return WW_KOKKOS_ERR_UNSUPPORTED;
+  }
+  if (ipar[WW_SRCE_I_SEED] != 0 || ipar[WW_SRCE_I_FLD2] != 0) {
+    report(g_last, WW_KOKKOS_ERR_UNSUPPORTED, "ww_srce_init: SEED and FLD2 are not supported");
+    return WW_KOKKOS_ERR_UNSUPPORTED;
+  }
+  const int nk = ipar[WW_SRCE_I_NK], nth = ipar[WW_SRCE_I_NTH], nspec = ipar[WW_SRCE_I_NSPEC];
+  if (nk <= 0 || nth <= 0 || nspec != nk * nth) {
+    report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww_srce_init: nspec != nk*nth");
+    return WW_KOKKOS_ERR_BAD_SHAPE;
+  }
+  srce::St4Dev q;
+  snl1::GridView dv;
+  if (!snl1::grid_view(igrid, dv) || !st4h::grid_state(igrid, q)) {
+    report(g_last, WW_KOKKOS_ERR_NOT_INITIALISED, "ww_srce_init before ww_snl1_init and ww_st4_commit for this grid");
+    return WW_KOKKOS_ERR_NOT_INITIALISED;
+  }
+  if (dv.g->nk != nk || dv.g->nth != nth || q.nk != nk || q.nth != nth) {
+    report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww_srce_init: NK/NTH differ from the DIA or ST4 context");
+    return WW_KOKKOS_ERR_BAD_SHAPE;
+  }
+  for (int is = 0; is < nspec; ++is)
+    if (mapwn[is] < 1 || mapwn[is] > nk) {
+      report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww_srce_init: MAPWN outside 1..NK");
+      return WW_KOKKOS_ERR_BAD_SHAPE;
+    }
+  Grid& g = grids()[igrid];
+  std::memcpy(g.r, rpar, sizeof g.r);
+  std::memcpy(g.ip, ipar, sizeof g.ip);
+  const size_t o_sig = 0, o_dden = al64(nk), o_ecos = o_dden + al64(nk), o_esin = o_ecos + al64(nspec);
+  const size_t o_map = o_esin + al64(nspec), words = o_map + al64(nspec);
+  std::vector<float> h(words, 0.f);
+  std::memcpy(&h[o_sig], sig, sizeof(float) * nk);
+  std::memcpy(&h[o_dden], dden, sizeof(float) * nk);
+  std::memcpy(&h[o_ecos], ecos, sizeof(float) * nspec);
+  std::memcpy(&h[o_esin], esin, sizeof(float) * nspec);
+  std::memcpy(&h[o_map], mapwn, sizeof(int) * nspec);
+  if (g.tables.ensure(words * sizeof(float)) != hipSuccess ||
+      hipMemcpy(g.tables.p, h.data(), words * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    release_grid(g);
+    grids().erase(igrid);
+    report(g_last, WW_KOKKOS_ERR_RUNTIME, "ww_srce_init: device copy");
+    return WW_KOKKOS_ERR_RUNTIME;
+  }
+  srce::SrceDev& d = g.dev;
+  const float* r = rpar;
+  d.nk = nk;
+  d.nth = nth;
+  d.nspec = nspec;
+  d.ic_numerics = ipar[WW_SRCE_I_IC_NUMERICS] != 0;
+  d.fdonly = ipar[WW_SRCE_I_FDONLY] != 0;
+  d.ln1 = ipar[WW_SRCE_I_LN1] != 0;
+  d.dmin = r[WW_SRCE_R_DMIN];
+  d.dtmax = r[WW_SRCE_R_DTMAX];
+  d.dtmin = r[WW_SRCE_R_DTMIN];
+  d.facti1 = r[WW_SRCE_R_FACTI1];
+  d.facti2 = r[WW_SRCE_R_FACTI2];
+  d.fachfa = r[WW_SRCE_R_FACHFA];
+  d.facp = r[WW_SRCE_R_FACP];
+  d.xflt = r[WW_SRCE_R_XFLT];
+  d.xrel = r[WW_SRCE_R_XREL];
+  d.fhmax = r[WW_SRCE_R_FHMAX];
+  d.ffxfm = r[WW_SRCE_R_FFXFM];
+  d.ffxpm = r[WW_SRCE_R_FFXPM];
+  d.ffxfa = r[WW_SRCE_R_FFXFA];
+  for (int j = 0; j < 4; ++j) d.icescales[j] = r[WW_SRCE_R_ICESCALES + j];
+  d.slnc1 = r[WW_SRCE_R_SLNC1];
+  d.fspm = r[WW_SRCE_R_FSPM];
+  d.fshf = r[WW_SRCE_R_FSHF];
+  d.sbtc1 = r[WW_SRCE_R_SBTC1];
+  d.sdbc1 = r[WW_SRCE_R_SDBC1];
+  d.sdbc2 = r[WW_SRCE_R_SDBC2];
+  const float* b = g.tables.as<float>();
+  d.sig = b + o_sig;
+  d.dden = b + o_dden;
+  d.ecos = b + o_ecos;
+  d.esin = b + o_esin;
+  d.mapwn = reinterpret_cast<const int*>(b + o_map);
+  g.ready = true;
+  report(g_last, WW_KOKKOS_OK, "");
+  return WW_KOKKOS_OK;
+}
+
+void ww_srce_batch(int igrid, int it, int inflags24, int flogrd57, int flogrd58, float dtg, int npts,
+                   float* spec, const float* wn1, const float* cg1, const float* pin, float* psc,
+                   float* whitecap, float* taubbl, float* tauice, int* shave) {
+  const srce::StepFlags f{it, inflags24 != 0, flogrd57 != 0, flogrd58 != 0, dtg};
+  (void)ww3gpu::srceh::batch_host(igrid, f, npts, spec, wn1, cg1, pin, psc, whitecap, taubbl, tauice, shave,
+                                  "ww_srce_batch");
+}
+
+void ww_srce(int igrid, int it, int inflags24, int flogrd57, int flogrd58, float* spec, const float* wn1,
+             const float* cg1, float d_inp, float u10abs, float u10dir, float as, float ice, float dtg,
+             float dair, float coef, float* ustar, float* ustdir, float* tauwx, float* tauwy, float* dtdyn,
+             float* fcut, float* tauox, float* tauoy, float* tauwix, float* tauwiy, float* tauwnx,
+             float* tauwny, float* phiaw, float* charn, float* tws, float* phioc, float* whitecap,
+             float* phibbl, float* taubbl, float* tauice, float* phice, float* tauocx, float* tauocy,
+             float* wnmean, float* fpi, int* shave) {
+  float* sc[WW_SRCE_NPSC] = {ustar, ustdir, tauwx,  tauwy,  dtdyn, fcut,   tauox,  tauoy,
+                             tauwix, tauwiy, tauwnx, tauwny, phiaw, charn,  tws,    phioc,
+                             phibbl, phice,  tauocx, tauocy, wnmean, fpi};
+  for (float* p : sc)
+    if (p == nullptr) {
+      ww3gpu::report(ww3gpu::srceh::g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww_srce: a null argument");
+      return;
+    }
+  const float pin[WW_SRCE_NPIN] = {d_inp, u10abs, u10dir, as, ice, dair, coef};
+  float psc[WW_SRCE_NPSC];
+  for (int j = 0; j < WW_SRCE_NPSC; ++j) psc[j] = *sc[j];
+  const srce::StepFlags f{it, inflags24 != 0, flogrd57 != 0, flogrd58 != 0, dtg};
+  int sh = shave != nullptr ? *shave : 0;
+  const int rc = ww3gpu::srceh::batch_host(igrid, f, 1, spec, wn1, cg1, pin, psc, whitecap, taubbl, tauice,
+                                           shave != nullptr ? &sh : nullptr, "ww_srce");
+  if (rc != WW_KOKKOS_OK) return;
+  for (int j = 0; j < WW_SRCE_NPSC; ++j) *sc[j] = psc[j];
+  *shave = sh;
+}
+
+int ww_srce_enabled(void) { return ww_kokkos_flag("WW_KOKKOS_SRCE"); }
+
+int ww_srce_last_error(void) { return ww3gpu::srceh::g_last.load(std::memory_order_relaxed); }
+
+int ww3gpu_srce_batch_dev(int igrid, int it, int inflags24, int flogrd57, int flogrd58, float dtg, int npts,
+                          float* spec, const float* wn1, const float* cg1, const float* pin, float* psc,
+                          float* whitecap, float* taubbl, float* tauice, int* shave, void* stream) {
+  using namespace ww3gpu::srceh;
+  const char* who = "ww3gpu_srce_batch_dev";
+  Grid* g = nullptr;
+  srce::St4Dev q;
+  snl1::GridView dv;
+  int rc = resolve(igrid, who, g, q, dv);
+  if (rc != WW_KOKKOS_OK) return rc;
+  if (!args_ok(npts, spec, wn1, cg1, pin, psc, whitecap, taubbl, tauice, shave)) {
+    report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww3gpu_srce_batch_dev: npts < 0 or a null array");
+    return WW_KOKKOS_ERR_BAD_SHAPE;
+  }
+  const srce::StepFlags f{it, inflags24 != 0, flogrd57 != 0, flogrd58 != 0, dtg};
+  if ((rc = check_step(q, it, who)) != WW_KOKKOS_OK) return rc;
+  if (npts == 0) {
+    report(g_last, WW_KOKKOS_OK, "");
+    return WW_KOKKOS_OK;
+  }
+  rc = run(*g, q, dv, f, npts, spec, wn1, cg1, pin, psc, whitecap, taubbl, tauice, shave,
+           static_cast<hipStream_t>(stream), who);
+  if (rc == WW_KOKKOS_OK) report(g_last, WW_KOKKOS_OK, "");
+  return rc;
+}
+
+int ww3gpu_srce_substeps(int igrid) {
+  ww3gpu::srceh::Grid* g = ww3gpu::srceh::find(igrid);
+  return g == nullptr ? -1 : g->substeps;
+}
+
+}  // extern "C"

@@ -50,6 +50,20 @@ SIGNATURES = {
     "ww3gpu_snl1_lds_bytes": (c_long, [c_int]),
     "ww3gpu_devlibm_eval": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_long, c_void_p]),
     "ww3gpu_version": (c_char_p, []),
+    "ww_st4_init": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int]),
+    "ww_st4_set_real": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p]),
+    "ww_st4_set_int": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p]),
+    "ww_st4_commit": (c_int, [c_int]),
+    "ww_st4_enabled": (c_int, [c_int]),
+    "ww_st4_last_error": (c_int, []),
+    "ww_srce_init": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int] + [c_void_p] * 5),
+    "ww_srce": (None, [c_int] * 5 + [c_void_p] * 3 + [c_float] * 8 + [c_void_p] * 26),
+    "ww_srce_batch": (None, [c_int] * 5 + [c_float, c_int] + [c_void_p] * 9),
+    "ww_srce_enabled": (c_int, []),
+    "ww_srce_last_error": (c_int, []),
+    "ww3gpu_srce_batch_dev": (c_int, [c_int] * 5 + [c_float, c_int] + [c_void_p] * 10),
+    "ww3gpu_srce_substeps": (c_int, [c_int]),
+    "ww3gpu_insin4_tables": (c_int, [c_void_p, c_int, c_void_p, c_int] + [c_void_p] * 7),
 }
 
 _lib = None
