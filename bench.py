@@ -1,12 +1,20 @@
 """Benchmark of the spectral-step hot path on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dia]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload srce|dia]
 
 A step is one pass of the hot path over one batch of synthetic sea points, with every
-input already resident in HBM.  Workload "dia": W3SNL1 over 9520 points, NK=32, NTH=36,
-log-uniform depths 10-4000 m (the point set and spectral grid of the full source-term
-step).  Prints one JSON line.  The CPU baseline is the oracle (a scalar C restatement,
-OpenMP over points) timed on a bounded sample on this host's cores.
+input already resident in HBM.
+
+Workload "srce" (default): the full source-term time step, W3SRCE with ST4 input and
+dissipation, the DIA, W3SLN1, W3SBT1, W3SDB1 and the semi-implicit sub-step integration to
+DTG = 300 s, over 9520 points, NK=32, NTH=36, U10 Weibull(k=2, mean 9 m/s) capped at
+35 m/s, depths log-uniform 10-4000 m, JONSWAP spectra.  Every timed step starts from the
+same input state (a device-to-device copy of the INOUT arrays, inside the timed region).
+
+Workload "dia": W3SNL1 alone over the same 9520 points.
+
+Prints one JSON line.  The CPU baseline is the oracle (a scalar C restatement, OpenMP over
+points) timed on a bounded sample on this host's cores.
 """
 from __future__ import annotations
 
@@ -137,12 +145,149 @@ def bench_dia(args):
     }
 
 
+# Bytes each kernel group must move per point it processes in one sub-step, for the roofline
+# of the dominant group (NSPEC-sized float arrays unless noted).  DIA: read A, write S_nl and
+# D_nl (and CG, KDMEAN).  Sub-step: read A, A_init, S_in, D_in, S_nl, D_nl, S_db and LLWS
+# (bytes); write A, S_in, D_in, LLWS, and S_ds, D_ds (written, then read back once).
+def srce_group_bytes(nk, nspec):
+    f = 4
+    return {
+        "DIA (W3SNL1)": f * (3 * nspec + nk + 1),
+        "sub-step (W3SLN1, W3SDS4, W3SDB1, W3SBT1, integration, W3SPR4, W3SIN4)":
+            f * (7 * nspec + 3 * nspec + 4 * nspec + 2 * nk) + 2 * nspec,
+    }
+
+
+SRCE_GROUPS = ["init (W3SRCE section 1, first W3SIN4)", "DIA (W3SNL1)",
+               "sub-step (W3SLN1, W3SDS4, W3SDB1, W3SBT1, integration, W3SPR4, W3SIN4)",
+               "finish (W3SRCE section 3)"]
+SRCE_KERNELS = ["init_kernel", "dia_fixed", "step_kernel", "finish_kernel"]
+SRCE_STEP_BYTES = 9744  # per grid-point-step: spec in+out, WN1/CG1, pin, psc in+out, whitecap..
+
+
+def bench_srce(args):
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from ww3gpu import capi, srce_case as sc
+    from ww3gpu.srce import Srce
+
+    nk, nth, npts = 32, 36, 9520
+    c = sc.make_case(npts, nk=nk, nth=nth, seed=2024)
+    op = Srce(1, c)
+    L = capi.lib()
+    fl = c["flags"]
+    keys = ("spec", "wn1", "cg1", "pin", "psc", "whitecap", "taubbl", "tauice", "shave")
+    src = {k: torch.from_numpy(np.ascontiguousarray(c[k])).cuda() for k in keys}
+    work = {k: v.clone() for k, v in src.items()}
+    inout = ("spec", "psc", "whitecap", "taubbl", "tauice", "shave")
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+
+    def step():
+        with torch.cuda.stream(stream):
+            for k in inout:
+                work[k].copy_(src[k])
+        return op.batch_dev(work, fl, stream)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    substeps = L.ww3gpu_srce_substeps(op.igrid)
+    check_timers = L.ww3gpu_srce_timing(op.igrid, 1)
+    capi.check("ww3gpu_srce_timing", check_timers)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_time(e1)
+    ms_step = gpu_ms / args.steps
+    value = npts * args.steps / (gpu_ms * 1e-3)
+    nt = len(SRCE_GROUPS)
+    ms = (ctypes.c_double * nt)()
+    nl = (ctypes.c_long * nt)()
+    capi.check("ww3gpu_srce_timers", L.ww3gpu_srce_timers(op.igrid, ctypes.addressof(ms),
+                                                            ctypes.addressof(nl), nt))
+    L.ww3gpu_srce_timing(op.igrid, 0)
+    groups = {SRCE_GROUPS[i]: {"kernel": SRCE_KERNELS[i], "ms_per_step": ms[i] / args.steps,
+                               "launches_per_step": nl[i] / args.steps,
+                               "avg_launch_us": 1e3 * ms[i] / max(nl[i], 1)} for i in range(nt)}
+    pts_substeps = L.ww3gpu_srce_point_substeps(op.igrid)
+    dom = max(groups, key=lambda g: groups[g]["ms_per_step"])
+    gb = srce_group_bytes(nk, nk * nth)
+    if dom in gb:
+        # bytes per launch: the group's bytes per point-sub-step times the point-sub-steps of
+        # one step, over the group's launches per step
+        per_launch = gb[dom] * pts_substeps / groups[dom]["launches_per_step"]
+        achieved = per_launch / (groups[dom]["avg_launch_us"] * 1e-6) / 1e9
+    else:
+        per_launch, achieved = None, None
+
+    # CPU baseline: the oracle's W3SRCE on a bounded sample of the same points.
+    import oracle as O
+    cores = cpu_cores()
+    tab = op.tables
+    sample = min(npts, cores * 32)
+    O.srce_batch(c, tab, nthreads=cores, npts=cores)  # warm
+    reps, c0 = 0, time.perf_counter()
+    while True:
+        O.srce_batch(c, tab, nthreads=cores, npts=sample)
+        reps += 1
+        if time.perf_counter() - c0 > args.cpu_seconds:
+            break
+    cpu_s = time.perf_counter() - c0
+    cpu_value = sample * reps / cpu_s
+
+    return {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grid-point-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded JONSWAP spectra, Weibull winds, log-uniform depths)",
+        "config": {
+            "workload": "full ST4 source-term step (W3SRCE: W3SIN4 + W3SDS4 + W3SNL1 + W3SLN1 + "
+                        "W3SBT1 + W3SDB1, semi-implicit sub-steps to DTG): 9520 points, NK=32, "
+                        "NTH=36, DTG=300 s, IT=1",
+            "npts": npts, "nk": nk, "nth": nth, "dtg_s": fl["dtg"], "parallelism": "single GPU",
+            "substeps_max": substeps, "point_substeps_per_step": pts_substeps,
+            "wall_s": wall, "kernel_groups": groups,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+            "kernel": groups[dom]["kernel"], "group": dom, "bytes_per_launch": per_launch,
+            "step_bytes_per_point": SRCE_STEP_BYTES,
+            "step_achieved_gbs": SRCE_STEP_BYTES * npts / (ms_step * 1e-3) / 1e9,
+        },
+        "cpu_baseline": {
+            "value": cpu_value, "unit": "grid-point-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{sample} of the {npts} points x {reps} repetitions, {cpu_s:.1f} s, "
+                      f"oracle W3SRCE (C, -O3, OpenMP over points)",
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="dia", choices=["dia"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="srce", choices=["srce", "dia"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -150,7 +295,14 @@ def main():
               "(source terms shard over points with no exchange; see the design notes)",
               file=sys.stderr)
         sys.exit(2)
-    res = bench_dia(args)
+    if args.workload == "dia":
+        args.steps = 500 if args.steps is None else args.steps
+        args.warmup = 50 if args.warmup is None else args.warmup
+        res = bench_dia(args)
+    else:
+        args.steps = 20 if args.steps is None else args.steps
+        args.warmup = 3 if args.warmup is None else args.warmup
+        res = bench_srce(args)
     print(json.dumps(res), flush=True)
 
 

@@ -249,6 +249,22 @@ int ww3gpu_srce_batch_dev(int igrid, int it, int inflags24, int flogrd57, int fl
  * largest over its points), or -1. */
 int ww3gpu_srce_substeps(int igrid);
 
+/* Point-sub-steps of the last call of grid igrid (the number of points that took part in
+ * each sub-step, summed over its sub-steps), or -1. */
+long ww3gpu_srce_point_substeps(int igrid);
+
+/* Per-kernel timing of grid igrid's W3SRCE calls.  enable = 1 starts it (and clears the
+ * timers), 0 stops it.  While on, every launch of a call is bracketed by HIP events on the
+ * call's stream.  Returns OK or NOT_INITIALISED. */
+int ww3gpu_srce_timing(int igrid, int enable);
+
+/* The timers since ww3gpu_srce_timing(igrid, 1), waiting for the recorded launches: ms[k]
+ * is the summed device time and launches[k] the number of launches of group k, for k < n:
+ * 0 pack (point-major to bin-major), 1 init, 2 input terms (W3SLN1, W3SIN4), 3 DIA,
+ * 4 dissipation and update (W3SDS4, W3SDB1, W3SBT1, limiter, update, W3SPR4, tail, W3SIN4),
+ * 5 finish, 6 unpack.  Returns OK, NOT_INITIALISED or BAD_SHAPE. */
+int ww3gpu_srce_timers(int igrid, double* ms, long* launches, int n);
+
 /* INSIN4's saturation and cumulative tables on the host, from the ST4 packets and the grid
  * arrays, as ww_st4_commit rebuilds them under WW_KOKKOS_ST4_TABLES=1
  * (kokkos/src/ww_kokkos/st4_insin4.cpp:110-138, 239-262): satw, sati (2*(MTH/2-1)+1, MTH),

@@ -213,10 +213,24 @@ def test_st4_commit_checks(rt):
     lb = np.array([1], np.int32)
     ex = np.array([10], np.int32)
     assert L.ww_st4_set_real(g, S.A_SIG, capi.ptr(a), 1, capi.ptr(lb), capi.ptr(ex)) == capi.Status.TABLE
+    # tabulated u* (SINTAILPAR(1) > 0.5): the commit first wants the module's stress tables
+    # (TABLE), then refuses the configuration (UNSUPPORTED)
     c["st4_rpar"][sc.ST4_R["SINTAILPAR"]] = 1.0
     g2 = _grid()
     S.register_st4(g2, c, tab)
+    assert L.ww_st4_commit(g2) == capi.Status.TABLE
+    for aid, ext in ((S.A_TAUT, [201, 201]), (S.A_TAUHFT, [101, 201]), (S.A_TAUHFT2, [101, 201, 51])):
+        z = np.zeros(int(np.prod(ext)), np.float32)
+        lb0 = np.zeros(len(ext), np.int32)
+        e = np.array(ext, np.int32)
+        assert L.ww_st4_set_real(g2, aid, capi.ptr(z), len(ext), capi.ptr(lb0), capi.ptr(e)) == capi.Status.OK
     assert L.ww_st4_commit(g2) == capi.Status.UNSUPPORTED
+    # SDSBCHOICE other than 1
+    c["st4_rpar"][sc.ST4_R["SINTAILPAR"]] = 0.0
+    c["st4_rpar"][sc.ST4_R["SSDSC"]] = 2.0
+    g3 = _grid()
+    S.register_st4(g3, c, tab)
+    assert L.ww_st4_commit(g3) == capi.Status.UNSUPPORTED
     # W3SRCE before the ST4 commit of its grid
     c3 = _variant(npts=2)
     r, i = c3["srce_rpar"], c3["srce_ipar"]

@@ -57,12 +57,13 @@ struct Layout {
   long cg_k_stride;   ///< cg: distance between frequencies
   long cg_pt_stride;  ///< cg: distance between points
   const int* skip;    ///< when set, points with skip[p] != 0 are left untouched
-  int xcd_chunk;      ///< > 0: workgroup i takes point (i % 8) * xcd_chunk + i / 8
+  int xcd_chunk;      ///< > 0: workgroup i takes slot (i % 8) * xcd_chunk + i / 8
+  const int* list;    ///< when set, slot i is point list[i] (npts counts the slots)
 };
 
 /// The C ABI's layout: a, s, d (NSPEC, npts) and cg (NK, npts), column-major.
 inline Layout point_major(int nk, int nspec) {
-  return Layout{1, nspec, 1, nk, nullptr, 0};
+  return Layout{1, nspec, 1, nk, nullptr, 0, nullptr};
 }
 
 /// Host-side result of INSNL1 for one grid.

@@ -1,14 +1,21 @@
 // ww3-gpu_amd/csrc/srce.hpp
 // ST4 source terms and the W3SRCE integration on gfx950: parameters, tables and the
-// resident per-point state.
+// per-point state of one call.
 //
 // Reference: W3SRCE kokkos/src/ww_kokkos/srce_kernel.hpp:212-610, ST4
 // kokkos/src/ww_kokkos/st4_*.hpp, packets st4_params.hpp:79-137 and srce_params.hpp:39-87.
 //
-// Layout.  The C ABI hands over point-major arrays (a point's NSPEC bins contiguous).  The
-// kernels run one point per lane, so the resident state is bin-major with the point index
-// fastest ("[bin][P]", P = npts padded to 64): every load of one bin by a wavefront of 64
-// points is one contiguous 256-byte line.  Transposes happen only at the C ABI boundary.
+// Work mapping.  A point is integrated by a group of 16 lanes (one DPP row; four points per
+// wavefront).  The lanes share a point's per-bin work (growth rates, saturation, the
+// cumulative breaking term, the update) and run its fixed-order sums together: every lane
+// of the row adds the row's values in bin order, each operand broadcast from the lane that
+// holds it (v_add_f32 with a row_newbcast DPP source), so each sum keeps the reference's
+// order and every lane ends with the same bits.
+//
+// Layout.  Everything per point is point-major, as the C ABI passes it: a point's NSPEC bins
+// are contiguous, so a row's 16 lanes read 64 consecutive bytes.  The caller's arrays are
+// used in place; the library adds the per-bin arrays that live across kernels (initial
+// spectrum, input and DIA terms, W3SDB1's source, LLWS) and a scalar block per point.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,6 +24,8 @@
 
 namespace ww3gpu {
 namespace srce {
+
+constexpr int kGroup = 16;  ///< lanes per point (one DPP row)
 
 /// ST4 scalars (W3GDATMD SRCP and W3SRC4MD module variables) and device table pointers.
 struct St4Dev {
@@ -34,7 +43,9 @@ struct St4Dev {
   const float* ecos;    ///< ECOS(1:NSPEC+NTH)
   const float* satw;    ///< SATWEIGHTS(1:NSAT,1:NTH)
   const int* sati;      ///< SATINDICES(1:NSAT,1:NTH)
-  const float* cumulw;  ///< CUMULW(1:NSPEC,1:NSPEC)
+  /// CUMULW(IS2,IS) as a function of (IK, IK2, |ITH2-ITH|): cumulk[((IK-1)*NK + IK2-1)*NTH + D]
+  /// (INSIN4's table has this structure; the commit checks every entry against it).
+  const float* cumulk;
   const float* fwtable; ///< FWTABLE(0:300), or null when SWELLF(2) == 0
 };
 
@@ -56,7 +67,7 @@ struct StepFlags {
   float dtg;
 };
 
-/// Per-point scalar fields of the resident state; the first 22 are the C ABI's psc rows.
+/// Per-point scalar fields; the first 22 are the C ABI's psc rows.
 enum Scalar : int {
   S_USTAR = 0, S_USTDIR, S_TAUWX, S_TAUWY, S_DTDYN, S_FCUT, S_TAUOX, S_TAUOY, S_TAUWIX,
   S_TAUWIY, S_TAUWNX, S_TAUWNY, S_PHIAW, S_CHARN, S_TWS, S_PHIOC, S_PHIBBL, S_PHICE, S_TAUOCX,
@@ -71,58 +82,60 @@ enum PinRow : int { P_D_INP = 0, P_U10ABS, P_U10DIR, P_AS, P_ICE, P_DAIR, P_COEF
 /// Per-point integer fields.
 enum IScalar : int { I_NSTEPS = 0, I_DONE, I_SHAVE, I_NSCUT, N_ISCAL };
 
-/// Device pointers of the resident state of one batch of npts points, leading dimension P.
+/// The arrays of one call over npts points.
 struct State {
-  int npts, P;
-  float* spec;      ///< [NSPEC][P]
-  float* specinit;  ///< [NSPEC][P]
-  float* vsin;      ///< [NSPEC][P]
-  float* vdin;
-  float* vsnl;
-  float* vdnl;
-  float* vsds;
-  float* vdds;
-  float* vsdb;      ///< W3SDB1's S, kept across sub-steps (its early return keeps it)
-  float* brlambda;
-  int* llws;        ///< [NSPEC][P]
-  float* wn1;       ///< [NK][P]
-  float* cg1;       ///< [NK][P]
-  float* bth0;      ///< [NK][P]  W3SDS4 saturation maxima
-  float* wnf;       ///< [NK][P]  W3SLN1 frequency factor
-  float* cbeta;     ///< [NK][P]  W3SBT1 coefficient
-  float* dirf;      ///< [NTH][P] W3SLN1 direction factor
-  float* sc;        ///< [N_SCAL][P]
-  int* isc;         ///< [N_ISCAL][P]
-  float* pin;       ///< [N_PIN][P]
-  float* whitecap;  ///< [4][P]
-  float* taubbl;    ///< [2][P]
-  float* tauice;    ///< [2][P]
-  float* kdmean;    ///< [P] the DIA's KDMEAN of the current sub-step
-  int* active;      ///< [kMaxIter] points still integrating after each sub-step
+  int npts;
+  // the caller's arrays (C ABI shapes, point-major)
+  float* spec;            ///< [npts][NSPEC] INOUT
+  const float* wn1;       ///< [npts][NK]
+  const float* cg1;       ///< [npts][NK]
+  const float* pin;       ///< [npts][N_PIN]
+  float* psc;             ///< [npts][N_PSC] INOUT
+  float* whitecap;        ///< [npts][4] INOUT
+  float* taubbl;          ///< [npts][2] INOUT
+  float* tauice;          ///< [npts][2] INOUT
+  int* shave;             ///< [npts] OUT
+  // the library's arrays
+  float* specinit;        ///< [npts][NSPEC]
+  float* vsin;            ///< [npts][NSPEC] W3SIN4's S (DSTAB(1) under STAB3 while it runs)
+  float* vdin;            ///< [npts][NSPEC] W3SIN4's D
+  float* vsnl;            ///< [npts][NSPEC] DIA S
+  float* vdnl;            ///< [npts][NSPEC] DIA D
+  float* vsds;            ///< [npts][NSPEC] W3SDS4's S of the current sub-step
+  float* vdds;            ///< [npts][NSPEC] W3SDS4's D of the current sub-step
+  float* vsdb;            ///< [npts][NSPEC] W3SDB1's S, kept across sub-steps (its early return)
+  unsigned char* llws;    ///< [npts][NSPEC]
+  float* sc;              ///< [npts][N_SCAL]
+  int* isc;               ///< [npts][N_ISCAL]
+  float* kdmean;          ///< [npts] the DIA's KDMEAN of the next sub-step
+  int* list[2];           ///< points still integrating, alternate sub-steps
+  int* count;             ///< [kMaxIter] length of the list each sub-step leaves
 };
 
-/// Bytes of device memory a State of npts points takes (with padding).
+/// Bytes of library memory a call over npts points needs (State minus the caller's arrays).
 size_t state_bytes(int nk, int nth, int npts);
-/// Carve a State out of `base` (state_bytes() long, 256-byte aligned).
+/// Carve the library's arrays of a State out of `base` (state_bytes() long, 256-aligned).
 State carve_state(void* base, int nk, int nth, int npts);
 
-/// The kernels (srce_kernels.hip).  Every launch covers the batch's npts points, one per
-/// lane; points that finished their integration return at once.
-hipError_t k_pack(const State& s, int nk, int nth, const float* spec, const float* wn1,
-                  const float* cg1, const float* pin, const float* psc, const float* whitecap,
-                  const float* taubbl, const float* tauice, hipStream_t st);
-hipError_t k_unpack(const State& s, int nk, int nth, float* spec, float* psc, float* whitecap,
-                    float* taubbl, float* tauice, int* shave, hipStream_t st);
+/// LDS bytes one workgroup (four points) of the group kernels takes.
+size_t group_lds_bytes(int nk, int nth);
+
+/// The kernels (srce_group.hip).
+/// init: W3SRCE's section 1 for every point, then the first sub-step's input terms and
+/// KDMEAN; lists every point in list[0] and clears the sub-step counters.
 hipError_t k_init(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f,
                   hipStream_t st);
-hipError_t k_step_a(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f,
-                    int iter, hipStream_t st);
-hipError_t k_step_c(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f,
-                    int iter, hipStream_t st);
+/// One sub-step for the n points of list[iter % 2] (after the DIA of that sub-step): the
+/// dissipation terms, the time-step limit, the update, the fluxes, W3SPR4, the tail, the
+/// next sub-step's input terms.  Points not done are appended to list[(iter + 1) % 2] and
+/// counted in count[iter].
+hipError_t k_step(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f,
+                  int iter, int n, hipStream_t st);
+/// W3SRCE's section 3 for every point and the outputs.
 hipError_t k_finish(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f,
                     hipStream_t st);
 
-/// Counter slots of State::active: the most sub-steps one call may take.  W3SRCE's
+/// Counter slots of State::count: the most sub-steps one call may take.  W3SRCE's
 /// sub-steps are at least min(DTMIN, rest) long, so a call takes about DTG/DTMIN of them;
 /// a call that would need more is stopped with an error instead of looping on.
 constexpr int kMaxIter = 4096;

@@ -270,7 +270,7 @@ int upload(Grid& g) {
   const size_t o_dden2 = take(nspec), o_esin = take(nspec + nth), o_ecos = take(nspec + nth);
   const size_t o_satw = take(static_cast<size_t>(nsat) * nth);
   const size_t o_sati = take(static_cast<size_t>(nsat) * nth);
-  const size_t o_cum = take(static_cast<size_t>(nspec) * nspec);
+  const size_t o_cum = take(static_cast<size_t>(nk) * nk * nth);
   std::vector<float> h(off, 0.f);
   auto put = [&](size_t o, const Arr& a, size_t n) { std::memcpy(&h[o], a.f.data(), n * sizeof(float)); };
   put(o_sig, g.a[WW_ST4_A_SIG], nk + 2);
@@ -282,10 +282,22 @@ int upload(Grid& g) {
   put(o_ecos, g.a[WW_ST4_A_ECOS], nspec + nth);
   put(o_satw, sw, static_cast<size_t>(nsat) * nth);
   std::memcpy(&h[o_sati], g.a[WW_ST4_A_SATINDICES].i.data(), sizeof(int) * static_cast<size_t>(nsat) * nth);
+  // CUMULW(IS2, IS) for IS = (IK, ITH), IS2 = (IK2, ITH2) depends on IK, IK2 and |ITH2 - ITH|
+  // only (INSIN4's construction): keep one row per (IK, IK2), read from ITH = 1, and check
+  // every registered entry against it.
   const std::vector<float>& cw = g.a[WW_ST4_A_CUMULW].f;
+  const auto at = [&](int is, int is2) { return cw[static_cast<size_t>(is2) + static_cast<size_t>(mspec) * is]; };
+  for (int ik = 0; ik < nk; ++ik)
+    for (int ik2 = 0; ik2 < nk; ++ik2)
+      for (int d = 0; d < nth; ++d)
+        h[o_cum + (static_cast<size_t>(ik) * nk + ik2) * nth + d] = at(ik * nth, ik2 * nth + d);
   for (int is = 0; is < nspec; ++is)
-    std::memcpy(&h[o_cum + static_cast<size_t>(is) * nspec], &cw[static_cast<size_t>(is) * mspec],
-                sizeof(float) * nspec);
+    for (int is2 = 0; is2 < nspec; ++is2) {
+      const int ik = is / nth, ith = is % nth, ik2 = is2 / nth, ith2 = is2 % nth;
+      const float v = at(is, is2);
+      const float k = h[o_cum + (static_cast<size_t>(ik) * nk + ik2) * nth + std::abs(ith2 - ith)];
+      if (std::memcmp(&v, &k, sizeof v) != 0) return WW_KOKKOS_ERR_UNSUPPORTED;
+    }
   if (g.tables.ensure(off * sizeof(float)) != hipSuccess ||
       hipMemcpy(g.tables.p, h.data(), off * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
     return WW_KOKKOS_ERR_RUNTIME;
@@ -343,7 +355,7 @@ int upload(Grid& g) {
   d.ecos = b + o_ecos;
   d.satw = b + o_satw;
   d.sati = reinterpret_cast<const int*>(b + o_sati);
-  d.cumulw = b + o_cum;
+  d.cumulk = b + o_cum;
   d.fwtable = nullptr;
   return WW_KOKKOS_OK;
 }
@@ -356,6 +368,11 @@ bool supported(const Grid& g, std::string& why) {
   }
   if (fnint(g.r[WW_ST4_R_SSDSC]) != 1) {
     why = "SDSBCHOICE = NINT(SSDSC(1)) other than 1 is not supported";
+    return false;
+  }
+  // The friction-factor lookup indexes FWTABLE with (LOG10(...) + 1) / DELAB.
+  if (g.r[WW_ST4_R_SSWELLF + 1] != 0.f && !(module().delab > 0.f)) {
+    why = "SWELLF(2) /= 0 needs DELAB > 0 for the FWTABLE lookup";
     return false;
   }
   return true;
@@ -390,6 +407,9 @@ using srce::St4Dev;
 using srce::State;
 using srce::StepFlags;
 
+/// Timer ids of ww3gpu_srce_timers: init, DIA, sub-step, finish.
+enum Timer : int { T_INIT = 0, T_DIA, T_STEP, T_FINISH, kTimers };
+
 struct Grid {
   bool ready = false;
   float r[WW_SRCE_NRPAR] = {};
@@ -400,6 +420,15 @@ struct Grid {
   DevBuf stage;   ///< device copies of a host-array call's arrays
   int* counter = nullptr;  ///< pinned: the host's copy of one sub-step's active count
   int substeps = -1;
+  long point_substeps = -1;  ///< sum over the last call's sub-steps of the points integrated
+  // Per-kernel timing (ww3gpu_srce_timing): event pairs around each launch on the call's
+  // stream, resolved when the timers are read.
+  bool timing = false;
+  std::vector<hipEvent_t> events;  ///< pool, reused call after call
+  size_t used = 0;                 ///< events of the pool recorded since the last resolve
+  std::vector<int> pending;        ///< timer id of each recorded pair
+  double ms[kTimers] = {};
+  long launches[kTimers] = {};
 };
 
 std::map<int, Grid>& grids() {
@@ -411,6 +440,10 @@ std::atomic<int> g_last{WW_KOKKOS_OK};
 bool g_hooked = false;
 
 void release_grid(Grid& g) {
+  for (hipEvent_t e : g.events) (void)hipEventDestroy(e);
+  g.events.clear();
+  g.used = 0;
+  g.pending.clear();
   g.tables.release();
   g.state.release();
   g.stage.release();
@@ -431,6 +464,36 @@ Grid* find(int igrid) {
 }
 
 size_t al64(size_t n) { return (n + 63) / 64 * 64; }
+
+/// Record the start (begin = true) or the end of a timed launch on `st`.
+void mark(Grid& g, int id, bool begin, hipStream_t st) {
+  if (!g.timing) return;
+  if (g.used == g.events.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) {
+      g.timing = false;
+      return;
+    }
+    g.events.push_back(e);
+  }
+  (void)hipEventRecord(g.events[g.used++], st);
+  if (begin) g.pending.push_back(id);
+}
+
+/// Wait for the recorded events and add their intervals to the timers.
+void resolve_timers(Grid& g) {
+  if (g.used == 0) return;
+  (void)hipEventSynchronize(g.events[g.used - 1]);
+  for (size_t k = 0; k < g.pending.size() && 2 * k + 1 < g.used; ++k) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g.events[2 * k], g.events[2 * k + 1]) == hipSuccess) {
+      g.ms[g.pending[k]] += ms;
+      g.launches[g.pending[k]] += 1;
+    }
+  }
+  g.used = 0;
+  g.pending.clear();
+}
 
 /// The context a compute call needs: W3SRCE grid, ST4 state, DIA view, consistent shapes.
 int resolve(int igrid, const char* who, Grid*& g, St4Dev& q, snl1::GridView& dv) {
@@ -465,45 +528,59 @@ int run(Grid& g, const St4Dev& q, const snl1::GridView& dv, const StepFlags& f, 
     report(g_last, WW_KOKKOS_ERR_RUNTIME, (std::string(who) + ": pinned allocation").c_str());
     return WW_KOKKOS_ERR_RUNTIME;
   }
-  const State s = srce::carve_state(g.state.p, nk, nth, npts);
+  State s = srce::carve_state(g.state.p, nk, nth, npts);
+  s.spec = spec;
+  s.wn1 = wn1;
+  s.cg1 = cg1;
+  s.pin = pin;
+  s.psc = psc;
+  s.whitecap = whitecap;
+  s.taubbl = taubbl;
+  s.tauice = tauice;
+  s.shave = shave;
   auto fail = [&](hipError_t e, const char* what) {
     std::string m = std::string(who) + ": " + what + ": " + hipGetErrorString(e);
     report(g_last, WW_KOKKOS_ERR_KERNEL, m.c_str());
     return WW_KOKKOS_ERR_KERNEL;
   };
-  hipError_t e = srce::k_pack(s, nk, nth, spec, wn1, cg1, pin, psc, whitecap, taubbl, tauice, st);
-  if (e != hipSuccess) return fail(e, "pack");
-  e = srce::k_init(s, q, g.dev, f, st);
+  resolve_timers(g);
+  mark(g, T_INIT, true, st);
+  hipError_t e = srce::k_init(s, q, g.dev, f, st);
+  mark(g, T_INIT, false, st);
   if (e != hipSuccess) return fail(e, "init");
-  // The DIA reads and writes the bin-major state in place; points already done are skipped.
-  // Workgroups are dealt to the 8 XCDs in contiguous runs of points, so the cache lines of
-  // one bin (64 consecutive points) stay in one XCD's L2.
-  const int chunk = (npts + 7) / 8;
-  const snl1::Layout L{s.P, 1, s.P, 1, s.isc + static_cast<size_t>(srce::I_DONE) * s.P, chunk};
-  int iter = 0;
+  int iter = 0, n = npts;
+  long pts = 0;
   for (;; ++iter) {
     if (iter == srce::kMaxIter) {
       report(g_last, WW_KOKKOS_ERR_KERNEL,
              (std::string(who) + ": the sub-step loop did not reach DTG within its step limit").c_str());
       return WW_KOKKOS_ERR_KERNEL;
     }
-    e = srce::k_step_a(s, q, g.dev, f, iter, st);
-    if (e != hipSuccess) return fail(e, "sub-step (input terms)");
-    e = snl1::launch(*dv.g, dv.af11_row, dv.sig, npts, s.spec, s.cg1, s.kdmean, s.vsnl, s.vdnl, st,
+    pts += n;
+    // The DIA of the points still integrating (the list the last sub-step left), dealt to
+    // the 8 XCDs in contiguous runs of list slots.
+    const snl1::Layout L{1, dv.g->nspec, 1, nk, nullptr, (n + 7) / 8, s.list[iter & 1]};
+    mark(g, T_DIA, true, st);
+    e = snl1::launch(*dv.g, dv.af11_row, dv.sig, n, s.spec, s.cg1, s.kdmean, s.vsnl, s.vdnl, st,
                      dv.variant, L);
+    mark(g, T_DIA, false, st);
     if (e != hipSuccess) return fail(e, "sub-step (DIA)");
-    e = srce::k_step_c(s, q, g.dev, f, iter, st);
-    if (e != hipSuccess) return fail(e, "sub-step (dissipation and update)");
-    e = hipMemcpyAsync(g.counter, s.active + iter, sizeof(int), hipMemcpyDeviceToHost, st);
+    mark(g, T_STEP, true, st);
+    e = srce::k_step(s, q, g.dev, f, iter, n, st);
+    mark(g, T_STEP, false, st);
+    if (e != hipSuccess) return fail(e, "sub-step");
+    e = hipMemcpyAsync(g.counter, s.count + iter, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(e, "sub-step count");
-    if (*g.counter == 0) break;
+    n = *g.counter;
+    if (n == 0) break;
   }
   g.substeps = iter + 1;
+  g.point_substeps = pts;
+  mark(g, T_FINISH, true, st);
   e = srce::k_finish(s, q, g.dev, f, st);
+  mark(g, T_FINISH, false, st);
   if (e != hipSuccess) return fail(e, "finish");
-  e = srce::k_unpack(s, nk, nth, spec, psc, whitecap, taubbl, tauice, shave, st);
-  if (e != hipSuccess) return fail(e, "unpack");
   return WW_KOKKOS_OK;
 }
 
@@ -681,7 +758,9 @@ int ww_st4_commit(int igrid) {
   }
   const int rc = upload(g);
   if (rc != WW_KOKKOS_OK) {
-    report(g_last, rc, "ww_st4_commit: device copy");
+    report(g_last, rc, rc == WW_KOKKOS_ERR_UNSUPPORTED
+                           ? "ww_st4_commit: CUMULW is not a function of (IK, IK2, |ITH2-ITH|)"
+                           : "ww_st4_commit: device copy");
     return rc;
   }
   g.ready = true;
@@ -766,6 +845,10 @@ int ww_srce_init(int igrid, const float* rpar, int nrpar, const int* ipar, int n
       report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww_srce_init: MAPWN outside 1..NK");
       return WW_KOKKOS_ERR_BAD_SHAPE;
     }
+  if (srce::group_lds_bytes(nk, nth) > 64u * 1024u) {
+    report(g_last, WW_KOKKOS_ERR_UNSUPPORTED, "ww_srce_init: spectrum too large for the kernels' LDS");
+    return WW_KOKKOS_ERR_UNSUPPORTED;
+  }
   Grid& g = grids()[igrid];
   std::memcpy(g.r, rpar, sizeof g.r);
   std::memcpy(g.ip, ipar, sizeof g.ip);
@@ -888,9 +971,38 @@ int ww3gpu_srce_batch_dev(int igrid, int it, int inflags24, int flogrd57, int fl
   return rc;
 }
 
+int ww3gpu_srce_timing(int igrid, int enable) {
+  ww3gpu::srceh::Grid* g = ww3gpu::srceh::find(igrid);
+  if (g == nullptr) return WW_KOKKOS_ERR_NOT_INITIALISED;
+  ww3gpu::srceh::resolve_timers(*g);
+  g->timing = enable != 0;
+  for (int k = 0; k < ww3gpu::srceh::kTimers; ++k) {
+    g->ms[k] = 0.0;
+    g->launches[k] = 0;
+  }
+  return WW_KOKKOS_OK;
+}
+
+int ww3gpu_srce_timers(int igrid, double* ms, long* launches, int n) {
+  ww3gpu::srceh::Grid* g = ww3gpu::srceh::find(igrid);
+  if (g == nullptr) return WW_KOKKOS_ERR_NOT_INITIALISED;
+  if (n < 0 || (n > 0 && (ms == nullptr || launches == nullptr))) return WW_KOKKOS_ERR_BAD_SHAPE;
+  ww3gpu::srceh::resolve_timers(*g);
+  for (int k = 0; k < n && k < ww3gpu::srceh::kTimers; ++k) {
+    ms[k] = g->ms[k];
+    launches[k] = g->launches[k];
+  }
+  return WW_KOKKOS_OK;
+}
+
 int ww3gpu_srce_substeps(int igrid) {
   ww3gpu::srceh::Grid* g = ww3gpu::srceh::find(igrid);
   return g == nullptr ? -1 : g->substeps;
+}
+
+long ww3gpu_srce_point_substeps(int igrid) {
+  ww3gpu::srceh::Grid* g = ww3gpu::srceh::find(igrid);
+  return g == nullptr ? -1 : g->point_substeps;
 }
 
 }  // extern "C"

@@ -1,0 +1,1460 @@
+// ww3-gpu_amd/csrc/srce_group.hip
+// W3SRCE with the ST4 source terms on gfx950: one 16-lane group (a DPP row) per point.
+//
+// Every per-point statement of the reference keeps its float32 operation order
+// (-ffp-contract=off), its sums in IK/ITH order, and its MIN/MAX operand order:
+//   W3SPR4 + CALC_USTAR  kokkos/src/ww_kokkos/st4_spr4.hpp:40-162, st4_calc_ustar.hpp:36-122
+//   W3SIN4               kokkos/src/ww_kokkos/st4_sin4.hpp:69-277, 472-627
+//   W3SDS4               kokkos/src/ww_kokkos/st4_sds4.hpp:122-331, 574-711
+//   W3SLN1, W3SBT1, W3SDB1  srce_ln1.hpp:41-99, srce_bt1.hpp:26-58, srce_db1.hpp:44-149
+//   W3SRCE               kokkos/src/ww_kokkos/srce_kernel.hpp:212-583
+//
+// Lanes and order.  The 16 lanes of a point share its per-bin work.  A sum the reference
+// forms in bin order is formed the same way here: each lane computes the terms of its bins,
+// then every lane adds the row's terms in bin order, lane k's term arriving through a
+// row_newbcast:k DPP operand, so all 16 lanes hold the same, reference-ordered result.
+// Per-bin work uses the flat mapping (bin b on lane b % 16) unless a band's directions must
+// be visited in order inside one sub-step of a scan (W3SIN4's stress scan, the tail copy),
+// which use the band mapping (direction ith on lane ith % 16).
+//
+// Shared data.  A point's spectrum lives in LDS (X) while its directional gathers run; the
+// same buffer then holds BRLAMBDA for the cumulative term, then the updated spectrum.  Lanes
+// exchange data through X, the point's small LDS block (W) and the point's global arrays;
+// all exchanges stay inside one wavefront, whose LDS and vector-memory operations complete in
+// program order, so a compiler barrier (wave_sync) is the only ordering needed.
+//
+// Sub-step structure (each kernel launch is one of these):
+//   k_init  W3SRCE section 1, then the first sub-step's input terms (W3SIN4) and KDMEAN.
+//   [DIA]   W3SNL1 of the points still integrating (snl1_*.hip).
+//   k_step  W3SLN1, W3SDS4, W3SDB1, W3SBT1, the time-step limit, the update, the fluxes,
+//           W3SPR4, FHIGH, the limiter of the mean energy, the tail, W3SIN4.  That last
+//           W3SIN4 is the next sub-step's first: nothing it reads changes in between under
+//           SINTAILPAR(4) > 0.5; otherwise W3SPR4 runs after it and the next sub-step's
+//           W3SIN4 runs again at the end of the same launch.
+//   k_finish W3SRCE section 3.
+//
+// Fused or reordered, same arithmetic: W3SLN1's S and W3SBT1's S and D are recomputed where
+// read; the isotropic sums of W3SDS4's cumulative term (SSDSC(3) > 0) are recomputed per
+// direction; sincosf(USDIR) of W3SDS4's turbulence loop is computed once; W3SDS4's
+// mean-square-slope block is skipped (nothing reads MSSSUM under SDSBCHOICE = 1); the
+// whitecap sums run after the cumulative term, which does not touch their inputs.  The
+// time-step limit's MIN over bins is formed in lane order: its operands are never NaN or
+// -0 (each is AFAC / MAX(., 1E-10) >= +0), so the order cannot change the result.
+#include "devlibm.hpp"
+#include "srce.hpp"
+
+namespace ww3gpu {
+namespace srce {
+namespace {
+
+constexpr int G = kGroup;
+constexpr int kPoints = 64 / G;  // points per workgroup (one wavefront)
+constexpr int kTailJ = 50;       // W3SIN4's tail integration steps
+constexpr int kTailArr = 7;      // precomputed tail arrays
+
+constexpr float kGrav = 9.806f;
+constexpr float kDwat = 1000.f;
+constexpr float kNuAir = 1.4E-5f;
+constexpr float kKappa = 0.40f;
+constexpr float kPi = static_cast<float>(3.141592653589793);
+constexpr float kTpi = 2.0f * kPi;
+constexpr float kTpiInv = 1.0f / kTpi;
+
+__device__ inline float mx(float a, float b) { return a > b ? a : b; }
+__device__ inline float mn(float a, float b) { return a < b ? a : b; }
+__device__ inline int mxi(int a, int b) { return a > b ? a : b; }
+__device__ inline int mni(int a, int b) { return a < b ? a : b; }
+__device__ inline float fmax2(float a, float b) { return (b > a) ? b : a; }
+__device__ inline float fmin2(float a, float b) { return (b < a) ? b : a; }
+__device__ inline int fint(float x) {
+  return (x > -2147483904.f && x < 2147483648.f) ? static_cast<int>(x)
+                                                 : static_cast<int>(0x80000000u);
+}
+__device__ inline int nint(float x) {
+  const float r = __builtin_roundf(x);
+  return (r > -9.2233715e18f && r < 9.2233715e18f) ? static_cast<int>(static_cast<long long>(r)) : 0;
+}
+__device__ inline int ffloor(float x) {
+  const int i = fint(x);
+  return static_cast<float>(i) <= x ? i : static_cast<int>(static_cast<unsigned>(i) - 1u);
+}
+
+/// Lane K's value of v, in every lane of its 16-lane row.
+template <int K>
+__device__ inline float bc(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + K, 0xF, 0xF, false));
+}
+
+/// Orders the lanes' LDS and global accesses around it for the compiler (the hardware keeps
+/// a wavefront's memory operations in order).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+#define WW_ROW16(F) F(0) F(1) F(2) F(3) F(4) F(5) F(6) F(7) F(8) F(9) F(10) F(11) F(12) F(13) F(14) F(15)
+
+/// Per-point LDS layout, in floats: X (NSPEC), then the small arrays.
+struct Lds {
+  int dirf, wnf, cbeta, bth0, sf, tail, small, stride;
+  __host__ __device__ Lds(int nk, int nth) {
+    dirf = 0;
+    wnf = dirf + nth;
+    cbeta = wnf + nk;
+    bth0 = cbeta + nk;
+    sf = bth0 + nk;
+    tail = sf + nk;
+    small = tail + kTailArr * (kTailJ + 2);
+    // X and W of the four points start 8 banks apart
+    stride = ((nk * nth + small + 31) / 32) * 32 + 8;
+  }
+};
+
+/// One point's view.
+struct Grp {
+  int p, l;
+  float* X;
+  float* W;
+  float* spec;
+  const float* wn1;
+  const float* cg1;
+  const float* pin;
+  float* specinit;
+  float* vsin;
+  float* vdin;
+  float* vsnl;
+  float* vdnl;
+  float* vsds;
+  float* vdds;
+  float* vsdb;
+  unsigned char* llws;
+  Lds o;
+};
+
+/// The point's scalars, in registers while a kernel runs.
+struct Sc {
+  float v[N_SCAL];
+  int iv[N_ISCAL];
+};
+
+__device__ inline Grp make_grp(const State& s, int nk, int nth, int p, float* lds, int g) {
+  Grp t{p, static_cast<int>(threadIdx.x) % G, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, Lds(nk, nth)};
+  const size_t nspec = static_cast<size_t>(nk) * nth;
+  t.X = lds + static_cast<size_t>(g) * t.o.stride;
+  t.W = t.X + nspec;
+  const size_t ps = static_cast<size_t>(p) * nspec;
+  t.spec = s.spec + ps;
+  t.wn1 = s.wn1 + static_cast<size_t>(p) * nk;
+  t.cg1 = s.cg1 + static_cast<size_t>(p) * nk;
+  t.pin = s.pin + static_cast<size_t>(p) * N_PIN;
+  t.specinit = s.specinit + ps;
+  t.vsin = s.vsin + ps;
+  t.vdin = s.vdin + ps;
+  t.vsnl = s.vsnl + ps;
+  t.vdnl = s.vdnl + ps;
+  t.vsds = s.vsds + ps;
+  t.vdds = s.vdds + ps;
+  t.vsdb = s.vsdb + ps;
+  t.llws = s.llws + ps;
+  return t;
+}
+
+__device__ inline void load_sc(const State& s, int p, Sc& c) {
+  const float* src = s.sc + static_cast<size_t>(p) * N_SCAL;
+  for (int i = 0; i < N_SCAL; ++i) c.v[i] = src[i];
+  const int* isrc = s.isc + static_cast<size_t>(p) * N_ISCAL;
+  for (int i = 0; i < N_ISCAL; ++i) c.iv[i] = isrc[i];
+}
+
+__device__ inline void store_sc(const State& s, const Grp& t, const Sc& c) {
+  if (t.l != 0) return;
+  float* dst = s.sc + static_cast<size_t>(t.p) * N_SCAL;
+  for (int i = 0; i < N_SCAL; ++i) dst[i] = c.v[i];
+  int* idst = s.isc + static_cast<size_t>(t.p) * N_ISCAL;
+  for (int i = 0; i < N_ISCAL; ++i) idst[i] = c.iv[i];
+}
+
+// ------------------------------------------------------------------------------------
+// CALC_USTAR (iterative branch, SINTAILPAR(1) < 0.5); uniform in the group
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ void calc_ustar(const St4Dev& q, float windspeed, float tauw, float& ustar, float& z0,
+                           float& charn) {
+  const float eps1 = 0.00001f;
+  float chath = q.aalpha, ust = 0.f, xmin;
+  if (q.capchnk[0] == 1.f) {
+    chath = q.capchnk[1] + 0.5f * (q.capchnk[2] - q.capchnk[1]) *
+                               (1.f - libm::tanhf((windspeed - q.capchnk[3]) / q.capchnk[4]));
+    xmin = 0.15f * (q.capchnk[2] - chath);
+  } else {
+    xmin = 0.f;
+  }
+  const float tl = tauw;
+  const float wcd = __builtin_sqrtf(0.0012875f);
+  const float ustold = windspeed * wcd;
+  float tauold = fmax2(ustold * ustold, tl + eps1);
+  constexpr float znunum = 0.11f * kNuAir;
+  constexpr float xm1 = 0.50f + 1.f;
+  for (int it = 1; it <= 10; ++it) {
+    const float x = fmax2(tl / tauold, xmin);
+    ust = __builtin_sqrtf(tauold);
+    float zz00 = chath * tauold / kGrav;
+    if (q.zz0max != 0.f) zz00 = fmin2(zz00, q.zz0max);
+    const float omx = 1.f - x;
+    float zz0 = zz00 / libm::powf(omx, 0.50f);
+    const float znu = znunum / fmax2(ust, 1E-6f);
+    zz0 = q.sintailpar[4] * znu + zz0;
+    const float lg = libm::logf(q.zzwnd / zz0);
+    const float fval = ust - kKappa * windspeed / lg;
+    const float delf = 1.f - kKappa * windspeed / (lg * lg) * 2.f / ust * (1.f - xm1 * x) / omx;
+    ust = ust - fval / delf;
+    tauold = fmax2(ust * ust, tl + eps1);
+  }
+  ustar = ust;
+  if (ustar > 0.001f) {
+    const float s = fmin2(windspeed / ustar, 100.0f);
+    z0 = q.zzwnd * libm::expf(-kKappa * s);
+    charn = kGrav * z0 / (ustar * ustar);
+  } else {
+    if (ustar > 0.f) {
+      const float s = fmin2(windspeed / ustar, 100.0f);
+      z0 = q.zzwnd * libm::expf(-kKappa * s);
+    } else {
+      z0 = chath * 0.001f * 0.001f / kGrav;
+    }
+    charn = chath;
+  }
+  if (q.capchnk[0] == 1.f) {
+    charn = fmin2(0.09f, charn);
+    if (charn < chath) charn = chath;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// W3SPR4 (reads the spectrum from X, LLWS from the point's array)
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ void spr4(const Grp& t, const St4Dev& q, Sc& c) {
+  const int nk = q.nk, nth = q.nth, nspec = q.nspec, l = t.l;
+  const float eps = 1.E-7f;
+  const float u = t.pin[P_U10ABS], udir = t.pin[P_U10DIR];
+  float ustar = mx(0.0001f, c.v[S_USTAR]);
+  float emean = 0.f, emeanws = 0.f, fmeanws = 0.f, fmean = 0.f, fmean1 = 0.f, wnm = 0.f;
+  float amax = 0.f, elcs = 0.f, elsn = 0.f;
+  float* sf = t.W + t.o.sf;
+  for (int ik = l; ik < nk; ik += G) sf[ik] = libm::powf(q.sig[ik + 1], q.ssdsc[11]) * q.dden[ik] / t.cg1[ik];
+  wave_sync();
+  // Section 1: ELCS, ELSN, AMAX over every bin in order.
+  for (int b0 = 0; b0 < nspec; b0 += G) {
+    const int b = b0 + l;
+    float pc = 0.f, ps = 0.f, aa = 0.f;
+    if (b < nspec) {
+      aa = t.X[b];
+      const float f = sf[b / nth];
+      pc = aa * q.ecos[b] * f;
+      ps = aa * q.esin[b] * f;
+    }
+    const int cnt = nspec - b0;
+#define ST(k)                                \
+  if (k < cnt) {                             \
+    elcs = elcs + bc<k>(pc);                 \
+    elsn = elsn + bc<k>(ps);                 \
+    amax = mx(amax, bc<k>(aa));              \
+  }
+    WW_ROW16(ST)
+#undef ST
+  }
+  const float dlw = libm::atan2f(elsn, elcs);
+  const float tw = 2.f * q.wwnmeanptail;
+  // Section 2: band sums per lane, then the band terms in IK order.
+  float elast = 0.f, e2last = 0.f;
+  for (int base = 0; base < nk; base += G) {
+    const int ik = base + l;
+    float e = 0.f, e2 = 0.f, tf = 0.f, tf1 = 0.f, twn = 0.f, tfw = 0.f;
+    if (ik < nk) {
+      float eb = 0.f, eb2 = 0.f;
+      const int is0 = ik * nth;
+      for (int ith = 0; ith < nth; ++ith) {
+        const float aa = t.X[is0 + ith];
+        eb = eb + aa;
+        if (t.llws[is0 + ith] != 0) eb2 = eb2 + aa;
+      }
+      const float pws = libm::powf(q.sig[ik + 1], tw);
+      const float pwk = libm::powf(t.wn1[ik], q.wwnmeanp);
+      const float cgk = t.cg1[ik];
+      e = eb * q.dden[ik] / cgk;
+      e2 = eb2 * q.dden[ik] / cgk;
+      tf = e / q.sig[ik + 1];
+      tf1 = e * pws;
+      twn = e * pwk;
+      tfw = e2 * pws;
+    }
+    const int cnt = nk - base;
+#define ST(k)                                \
+  if (k < cnt) {                             \
+    emean = emean + bc<k>(e);                \
+    fmean = fmean + bc<k>(tf);               \
+    fmean1 = fmean1 + bc<k>(tf1);            \
+    wnm = wnm + bc<k>(twn);                  \
+    emeanws = emeanws + bc<k>(e2);           \
+    fmeanws = fmeanws + bc<k>(tfw);          \
+  }
+    WW_ROW16(ST)
+#undef ST
+    if (base + G >= nk) {
+      elast = __shfl(e, nk - 1 - base, G);
+      e2last = __shfl(e2, nk - 1 - base, G);
+    }
+  }
+  float eband = elast / q.dden[nk - 1];
+  emean = emean + eband * q.fte;
+  fmean = fmean + eband * q.ftf;
+  fmean1 = fmean1 + eband * q.sstxftftail;
+  wnm = wnm + eband * q.sstxftwn;
+  eband = e2last / q.dden[nk - 1];
+  emeanws = emeanws + eband * q.fte;
+  fmeanws = fmeanws + eband * q.sstxftftail;
+  fmean = kTpiInv * emean / mx(eps, fmean);
+  if (fmean1 < eps)
+    fmean1 = kTpiInv * q.sig[nk];
+  else
+    fmean1 = kTpiInv * libm::powf(mx(eps, fmean1) / mx(eps, emean), 1.f / (2.f * q.wwnmeanptail));
+  wnm = libm::powf(mx(eps, wnm) / mx(eps, emean), 1.f / q.wwnmeanp);
+  if (fmeanws < eps || emeanws < eps)
+    fmeanws = kTpiInv * q.sig[nk];
+  else
+    fmeanws = kTpiInv * libm::powf(mx(eps, fmeanws) / mx(eps, emeanws), 1.f / (2.f * q.wwnmeanptail));
+  const float tx = c.v[S_TAUWX], ty = c.v[S_TAUWY];
+  const float tauw = __builtin_sqrtf(tx * tx + ty * ty);
+  float z0 = 0.f, charn = c.v[S_CHARN];
+  calc_ustar(q, u, tauw, ustar, z0, charn);
+  const float r = ustar / mx(0.01f, u);
+  c.v[S_USTAR] = ustar;
+  c.v[S_USTDIR] = udir;
+  c.v[S_EMEAN] = emean;
+  c.v[S_FMEAN] = fmean;
+  c.v[S_FMEAN1] = fmean1;
+  c.v[S_WNMEAN] = wnm;
+  c.v[S_AMAX] = amax;
+  c.v[S_CD] = r * r;
+  c.v[S_Z0] = z0;
+  c.v[S_CHARN] = charn;
+  c.v[S_FMEANWS] = fmeanws;
+  c.v[S_DLWMEAN] = dlw;
+}
+
+// ------------------------------------------------------------------------------------
+// W3SIN4 (W3SRCE's call: A = SPEC, K = WN2, U = U10ABS, USDIR = U10DIR); S, D into VSIN,
+// VDIN; TAUWNX/Y into TAUWAX/Y.  Band mapping for the per-bin arrays.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
+  const int nk = q.nk, nth = q.nth, l = t.l;
+  const float u = t.pin[P_U10ABS], usdir = t.pin[P_U10DIR], as = t.pin[P_AS];
+  const float ustar = c.v[S_USTAR], drat = c.v[S_DRAT], z0 = c.v[S_Z0];
+  constexpr float kappa2 = kKappa * kKappa;
+  constexpr float z0viscnum = 0.1f * kNuAir;
+  constexpr float twonu = 2.f * kNuAir;
+  constexpr float grav2tpi = (kGrav * kGrav) * kTpi;
+  constexpr float logtpi005 = 7.1361942291259765625f;  // LOG(TPI/0.005) as folded
+  constexpr float eps1 = 0.00001f, eps2 = 0.000001f, eps2sq = eps2 * eps2;
+  const float const1 = q.bbeta / kappa2;
+  float const0 = const1 * drat;
+  const float z0visc = z0viscnum / mx(ustar, 0.0001f);
+  const float z0noz = mx(z0visc, q.zz0rat * z0);
+  float uorb = 0.f, aorb = 0.f;
+  for (int base = 0; base < nk; base += G) {
+    const int ik = base + l;
+    float tu = 0.f, ta = 0.f;
+    if (ik < nk) {
+      float eb = 0.f;
+      for (int ith = 0; ith < nth; ++ith) eb = eb + t.X[ik * nth + ith];
+      const float sk = q.sig[ik + 1];
+      const float cgk = t.cg1[ik];
+      tu = eb * (sk * sk) * q.dden[ik] / cgk;
+      ta = eb * q.dden[ik] / cgk;
+    }
+    const int cnt = nk - base;
+#define ST(k)                      \
+  if (k < cnt) {                   \
+    uorb = uorb + bc<k>(tu);       \
+    aorb = aorb + bc<k>(ta);       \
+  }
+    WW_ROW16(ST)
+#undef ST
+  }
+  uorb = 2.f * __builtin_sqrtf(uorb);
+  const float aorb1 = 2.f * libm::powf(aorb, 1.f - 0.5f * q.sswellf[5]);
+  const float re = 4.f * uorb * aorb1 / kNuAir;
+  float pturb, pvisc;
+  if (q.sswellf[3] > 0.f) {
+    if (q.sswellf[6] > 0.f) {
+      const float smooth = 0.5f * libm::tanhf((re - q.sswellf[3]) / q.sswellf[6]);
+      pturb = 0.5f + smooth;
+      pvisc = 0.5f - smooth;
+    } else if (re <= q.sswellf[3]) {
+      pturb = 0.f;
+      pvisc = 1.f;
+    } else {
+      pturb = 1.f;
+      pvisc = 0.f;
+    }
+  } else {
+    pturb = 1.f;
+    pvisc = 1.f;
+  }
+  float fw, fu, fud;
+  if (q.sswellf[1] == 0.f) {
+    fw = __builtin_fabsf(q.sswellf[2]);
+    fu = 0.f;
+    fud = 0.f;
+  } else {
+    fu = __builtin_fabsf(q.sswellf[2]);
+    fud = q.sswellf[1];
+    aorb = 2.f * __builtin_sqrtf(aorb);
+    const float xi = (libm::log10f(mx(aorb / z0noz, 3.f)) - (-1.f)) / q.delab;
+    const int ind = mni(300 - 1, fint(xi));
+    const float deli1 = mn(1.f, xi - static_cast<float>(ind));
+    const float deli2 = 1.f - deli1;
+    fw = q.fwtable[ind] * deli2 + q.fwtable[ind + 1] * deli1;
+  }
+  float ustarsigma = 0.f;
+  if (q.stab3) {
+    const float usigma = (as != as) ? 0.f : mx(0.f, -0.025f * as);
+    ustarsigma = (1.0f + u / (10.f + u)) * usigma;
+  }
+
+  // The stress scan.  DSTAB(1, IS) (STAB3) is kept in VSIN until the average is formed.
+  float ust = ustar, taux = 0.f, tauy = 0.f, cosu = 0.f, sinu = 0.f, ustp = 0.f, usdirp = 0.f;
+  float s1x = 0.f, s1y = 0.f, n1x = 0.f, n1y = 0.f;  // istab 1
+  float s2x = 0.f, s2y = 0.f, n2x = 0.f, n2y = 0.f;  // istab 2
+  float s3x = 0.f, s3y = 0.f, n3x = 0.f, n3y = 0.f;  // istab 3
+  const int i0 = q.stab3 ? 1 : 3, i1 = q.stab3 ? 2 : 3;
+  for (int istab = i0; istab <= i1; ++istab) {
+    float* const dst = (istab == 1) ? t.vsin : t.vdin;
+    if (istab == 1) ust = ustar * (1.f - ustarsigma);
+    if (istab == 2) ust = ustar * (1.f + ustarsigma);
+    float cu, su;
+    libm::sincosf(usdir, &su, &cu);
+    taux = (ust * ust) * cu;
+    tauy = (ust * ust) * su;
+    float ssx = 0.f, ssy = 0.f, snx = 0.f, sny = 0.f;
+    for (int ik = 1; ik <= nk; ++ik) {
+      const float taupx = taux - __builtin_fabsf(q.ttauwshelter) * ssx;
+      const float taupy = tauy - __builtin_fabsf(q.ttauwshelter) * ssy;
+      ustp = mn(libm::powf((taupx * taupx + taupy * taupy), 0.25f), mx(ust, 0.3f));
+      usdirp = libm::atan2f(taupy, taupx);
+      libm::sincosf(usdirp, &sinu, &cosu);
+      const int is1 = 1 + (ik - 1) * nth;
+      const float kis = t.wn1[ik - 1];
+      const float s2 = q.sig2[is1 - 1];
+      const float cm = kis / s2;
+      const float ucn = ustp * cm + q.zzalp;
+      const float const2 = q.dden2[is1 - 1] / t.cg1[ik - 1] * kGrav / (q.sig[ik] / kis * drat);
+      const float cnst = s2 * const0;
+      const float zcn = libm::logf(kis * z0);
+      const float swv = -(q.sswellf[4] * drat * 2.f * kis * __builtin_sqrtf(twonu * s2));
+      const float swt = -(drat * q.sswellf[0] * 16.f * (s2 * s2) / kGrav);
+      for (int base = 0; base < nth; base += G) {
+        const int ith = base + l;
+        float dv = 0.f, temp2 = 0.f;
+        if (ith < nth) {
+          const int is = ith + (ik - 1) * nth;  // 0-based
+          const float ec = q.ecos[is], es = q.esin[is];
+          const float coswind = (ec * cosu + es * sinu);
+          int lw;
+          if (coswind > 0.01f) {
+            const float x = coswind * ucn;
+            const float zarg = kKappa / x;
+            const float zlog = zcn + zarg;
+            if (zlog < 0.f) {
+              const float zl2 = zlog * zlog;
+              dv = cnst * libm::expf(zlog) * (zl2 * zl2) * ucn * ucn * libm::powf(coswind, q.ssinthp);
+              lw = 1;
+            } else {
+              dv = 0.f;
+              lw = 0;
+            }
+            if (28.f * cm * ustar * coswind >= 1.f) lw = 1;
+          } else {
+            dv = 0.f;
+            lw = 0;
+          }
+          if ((q.sswellf[0] != 0.f && dv < 1E-7f * s2) || q.sswellf[2] > 0.f) {
+            const float dvisc = swv;
+            const float dturb = swt * (fw * uorb + (fu + fud * coswind) * ustp);
+            dv = dv + pturb * dturb + pvisc * dvisc;
+          }
+          t.llws[is] = static_cast<unsigned char>(lw);
+          dst[is] = dv;
+          temp2 = const2 * dv * t.X[is];
+        }
+        const int cnt = nth - base;
+        const int isb = base + (ik - 1) * nth;
+#define ST(k)                                       \
+  if (k < cnt) {                                    \
+    const float t2 = bc<k>(temp2);                  \
+    const float dk = bc<k>(dv);                     \
+    const float eck = q.ecos[isb + k];              \
+    const float esk = q.esin[isb + k];              \
+    if (dk < 0.f) {                                 \
+      snx = snx + t2 * eck;                         \
+      sny = sny + t2 * esk;                         \
+    } else {                                        \
+      ssx = ssx + t2 * eck;                         \
+      ssy = ssy + t2 * esk;                         \
+    }                                               \
+  }
+        WW_ROW16(ST)
+#undef ST
+      }
+    }
+    if (istab == 1) { s1x = ssx; s1y = ssy; n1x = snx; n1y = sny; }
+    if (istab == 2) { s2x = ssx; s2y = ssy; n2x = snx; n2y = sny; }
+    if (istab == 3) { s3x = ssx; s3y = ssy; n3x = snx; n3y = sny; }
+  }
+  float xstress, ystress, tauwnx, tauwny;
+  if (!q.stab3) {
+    xstress = s3x;
+    ystress = s3y;
+    tauwnx = n3x;
+    tauwny = n3y;
+  } else {
+    xstress = 0.5f * (s1x + s2x);
+    ystress = 0.5f * (s1y + s2y);
+    tauwnx = 0.5f * (n1x + n2x);
+    tauwny = 0.5f * (n1y + n2y);
+  }
+  // D (the STAB3 average) and S = D*A, on the lanes that wrote them.
+  for (int ik = 0; ik < nk; ++ik)
+    for (int ith = l; ith < nth; ith += G) {
+      const int is = ith + ik * nth;
+      if (q.stab3) t.vdin[is] = 0.5f * (t.vsin[is] + t.vdin[is]);
+      t.vsin[is] = t.vdin[is] * t.X[is];
+    }
+  {
+    const float taupx = taux - __builtin_fabsf(q.ttauwshelter) * xstress;
+    const float taupy = tauy - __builtin_fabsf(q.ttauwshelter) * ystress;
+    ustp = libm::powf((taupx * taupx + taupy * taupy), 0.25f);
+    usdirp = libm::atan2f(taupy, taupx);
+  }
+  ust = ustp;
+  {
+    const float sg = q.sig[nk], s2 = sg * sg, s5 = s2 * (s2 * sg);
+    const0 = q.dth * s5 / grav2tpi * kTpi * sg / t.cg1[nk - 1];
+  }
+  float temp = 0.f;
+  for (int base = 0; base < nth; base += G) {
+    const int ith = base + l;
+    float term = 0.f;
+    if (ith < nth) {
+      const int is = ith + (nk - 1) * nth;
+      const float cw = (q.ecos[is] * cosu + q.esin[is] * sinu);
+      const float cp = mx(cw, 0.f);
+      term = t.X[is] * (cp * (cp * cp));
+    }
+    const int cnt = nth - base;
+#define ST(k) \
+  if (k < cnt) temp = temp + bc<k>(term);
+    WW_ROW16(ST)
+#undef ST
+  }
+  const float levtail0 = const0 * temp;
+  // The tail integral: the terms independent of the running u* first, one step per lane.
+  const float sg = q.sig[nk];
+  const float zinf = libm::logf(sg * sg / kGrav);
+  const float dely = (logtpi005 - zinf) / 50.f;
+  const float sp2 = q.sintailpar[2];
+  constexpr int TS = kTailJ + 2;
+  float* tl = t.W + t.o.tail;  // cm, cgtail, g0, c1, fm, a0, a1
+  for (int j0 = l; j0 < kTailJ; j0 += G) {
+    const int j = j0 + 1;
+    const float y = libm::expf(zinf + static_cast<float>(j - 1) * dely);
+    const float ykm = (y / 363.f);
+    const float ykm2 = ykm * ykm;
+    const float tensk = 1.f + ykm2;
+    const float cm = __builtin_sqrtf(kGrav * tensk / y);
+    tl[0 * TS + j0] = cm;
+    tl[1 * TS + j0] = 0.5f * (3.f * ykm2 + 1.f) * __builtin_sqrtf(kGrav / (y * tensk));
+    tl[2 * TS + j0] = kGrav * z0 / (cm * cm);
+    if (sp2 > 0.f) {
+      const float ykm1 = (ykm - 1.f);
+      tl[3 * TS + j0] = 0.2325f / cm;
+      tl[4 * TS + j0] = libm::expf(-0.25f * (ykm1 * ykm1));
+      const float th = libm::tanhf((y - 20.f) / 5.f);
+      tl[5 * TS + j0] = levtail0 * 0.5f * (1.f - th);
+      tl[6 * TS + j0] = sp2 * 0.5f * (1.f + th);
+    }
+  }
+  wave_sync();
+  float tauhf;
+  {
+    float ustr = ust;
+    float taur = ust * ust, tau1 = 0.f;
+    for (int j = 1; j <= kTailJ; ++j) {
+      const float wj = (j == 1 || j == kTailJ) ? 0.5f : 1.f;
+      const float cm = tl[0 * TS + j - 1];
+      const float cgtail = tl[1 * TS + j - 1];
+      const float zx = ustr / cm + q.zzalp;
+      const float zarg = mn(kKappa / zx, 20.f);
+      const float zmu = mn(tl[2 * TS + j - 1] * libm::expf(zarg), 1.f);
+      const float zlog = mn(libm::logf(zmu), 0.f);
+      const float zl2 = zlog * zlog;
+      const float zbeta = const1 * zmu * (zl2 * zl2);
+      float levtail;
+      if (sp2 > 0.f) {
+        float alpham;
+        if (ustr < cm)
+          alpham = mx(0.f, 0.01f * (1.f + libm::logf(ustr / cm)));
+        else
+          alpham = 0.01f * (1.f + 3.f * libm::logf(ustr / cm));
+        const float alphat = alpham * tl[3 * TS + j - 1] * tl[4 * TS + j - 1];
+        levtail = tl[5 * TS + j - 1] + tl[6 * TS + j - 1] * alphat;
+      } else {
+        levtail = levtail0;
+      }
+      tau1 = tau1 + wj * zbeta * (ustr * ustr) * levtail * dely * cgtail / cm;
+      taur = taur - wj * q.sintailpar[1] * (ustr * ustr) * zbeta * levtail * dely * cgtail / cm;
+      ustr = __builtin_sqrtf(mx(taur, 0.f));
+    }
+    tauhf = tau1;
+  }
+  float cp, sp;
+  libm::sincosf(usdirp, &sp, &cp);
+  float tauwx = xstress + tauhf * cp;
+  float tauwy = ystress + tauhf * sp;
+  const float tauw = __builtin_sqrtf(tauwx * tauwx + tauwy * tauwy);
+  const float u2b = mx(ustar, eps2);
+  const float u2 = u2b * u2b;
+  const float tauwb = mn(tauw, mx(u2 - eps1, eps2sq));
+  if (tauwb < tauw) {
+    tauwx = tauwx * tauwb / tauw;
+    tauwy = tauwy * tauwb / tauw;
+  }
+  c.v[S_TAUWX] = tauwx;
+  c.v[S_TAUWY] = tauwy;
+  c.v[S_TAUWAX] = tauwnx;
+  c.v[S_TAUWAY] = tauwny;
+  wave_sync();
+}
+
+// ------------------------------------------------------------------------------------
+// W3SDS4 (W3SRCE's call: A = SPEC, K = WN1, USDIR = USTDIR), SDSBCHOICE = 1.  On entry X
+// holds the spectrum; on return X holds BRLAMBDA, VSDS/VDDS hold SRHS/DDIAG, W's bth0
+// the saturation maxima.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFlags& f, Sc& c, float* whitecap) {
+  const int nk = q.nk, nth = q.nth, nspec = q.nspec, l = t.l;
+  const float* cs = q.ssdsc;
+  const float ustar = c.v[S_USTAR], usdir = c.v[S_USTDIR], depth = c.v[S_DEPTH];
+  const float dair = t.pin[P_DAIR];
+  const float facturb = (((cs[4] * (ustar * ustar)) / kGrav) * dair) / kDwat;
+  const int dikcumul = nint(q.ssdsbrf1 / (q.xfr - 1.f));
+  const int ik1 = q.ig1 ? nint(q.igpars5) + 1 : 1;
+  constexpr float twopisq = 2.f * (kPi * kPi);
+  const float epsr = __builtin_sqrtf(q.ssdsbr);
+  const bool iso = q.ssdsdth >= 180.f;
+  float* bth0w = t.W + t.o.bth0;
+
+  // Pass 1: B per bin (kept in VSDS), and per band the sum (isotropic case) or MAXVAL.
+  float lim = -__builtin_inff(), asum = 0.f;
+  bool found = false;
+  for (int b0 = 0; b0 < nspec; b0 += G) {
+    const int b = b0 + l;
+    float av = 0.f, bv = 0.f;
+    if (b < nspec) {
+      const int ik0 = b / nth;
+      if (ik0 + 1 >= ik1) {
+        av = t.X[b];
+        if (!iso) {
+          const int ith = b - ik0 * nth;
+          const float kk = t.wn1[ik0];
+          const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;
+          const float* w = q.satw + q.nsat * ith;
+          const int* x = q.sati + q.nsat * ith;
+          const float* xb = t.X + ik0 * nth - 1;
+          float acc = 0.f;
+          for (int j = 0; j < q.nsat; ++j) acc = acc + w[j] * xb[x[j]];
+          bv = acc * facsat;
+          t.vsds[b] = bv;
+        }
+      }
+    }
+    const int cnt = nspec - b0;
+#define ST(k)                                                                   \
+  if (k < cnt) {                                                                \
+    const int bin = b0 + k;                                                     \
+    const int ik0 = bin / nth;                                                  \
+    const int ith = bin - ik0 * nth;                                            \
+    if (ik0 + 1 >= ik1) {                                                       \
+      if (ith == 0) {                                                           \
+        lim = -__builtin_inff();                                                \
+        found = false;                                                          \
+        asum = 0.f;                                                             \
+      }                                                                         \
+      if (iso) {                                                                \
+        asum = asum + bc<k>(av);                                                \
+      } else {                                                                  \
+        const float bk = bc<k>(bv);                                             \
+        if (!found) {                                                           \
+          if (bk >= lim) {                                                      \
+            lim = bk;                                                           \
+            found = true;                                                       \
+          }                                                                     \
+        } else if (bk > lim) {                                                  \
+          lim = bk;                                                             \
+        }                                                                       \
+      }                                                                         \
+      if (ith == nth - 1 && l == 0) {                                           \
+        const float kk = t.wn1[ik0];                                            \
+        const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;       \
+        bth0w[ik0] = iso ? asum * facsat : (found ? lim : __builtin_nanf(""));  \
+      }                                                                         \
+    }                                                                           \
+  }
+    WW_ROW16(ST)
+#undef ST
+  }
+  for (int ik0 = l; ik0 < ik1 - 1 && ik0 < nk; ik0 += G) bth0w[ik0] = 0.f;
+  wave_sync();
+
+  // Pass 2: DDIAG, PB -> BRLAMBDA (into X), SRHS = DDIAG*A.
+  for (int b = l; b < nspec; b += G) {
+    const int ik0 = b / nth;
+    const float a = t.X[b];
+    if (ik0 + 1 < ik1) {
+      t.vdds[b] = 0.f;
+      t.vsds[b] = 0.f * a;  // SRHS = DDIAG * A with DDIAG = 0
+      t.X[b] = 0.f;         // BTH = 0 there, so PB = MAX(-EPSR, 0)**2 * 28.16 = 0
+      continue;
+    }
+    const float kk = t.wn1[ik0];
+    const float bth0 = bth0w[ik0];
+    float miche;
+    if (q.ssdsbm[0] == 1.f) {
+      miche = 1.f;
+    } else {
+      const float x = libm::tanhf(mn(kk * depth, 10.f));
+      const float tt = x * (q.ssdsbm[1] + x * (q.ssdsbm[2] + x * (q.ssdsbm[3] + x * q.ssdsbm[4])));
+      miche = tt * tt;
+    }
+    const float coef1 = q.ssdsbr * miche;
+    const float m = mx(bth0 / coef1 - 1.f, 0.f);
+    const float sigk = q.sig[ik0 + 1];
+    const float sdiagiso = ((cs[1] * sigk) * cs[5]) * (m * m);
+    const float coef2 = ((cs[1] * sigk) * (1.f - cs[5])) / (coef1 * coef1);
+    const float bb = iso ? bth0 : t.vsds[b];
+    const float dd = sdiagiso + coef2 * libm::powf(mx(bb - coef1, 0.f), q.ssdsp);
+    const float mm = mx(__builtin_sqrtf(bb) - epsr, 0.f);
+    const float pb = (mm * mm) * 28.16f;
+    t.vdds[b] = dd;
+    t.X[b] = pb / twopisq;
+    t.vsds[b] = dd * a;
+  }
+  wave_sync();
+
+  // Pass 3: wave-turbulence interaction and the cumulative effect.
+  if (cs[2] != 0.f || cs[4] != 0.f || cs[20] != 0.f) {
+    float su, cu;
+    libm::sincosf(usdir, &su, &cu);
+    for (int b = l; b < nspec; b += G) {
+      const int ik0 = b / nth;
+      const int ik = ik0 + 1;
+      if (ik < ik1) continue;
+      const int ith = b - ik0 * nth;
+      const float kk = t.wn1[ik0];
+      const float facturb2 = -(((q.sig[ik] * 2.f) * kk) * facturb);
+      const float dvisc = -(((cs[20] * 4.f) * kk) * kk);
+      const float cc = q.sig[ik] / kk;
+      float renew = 0.f;
+      if (cs[2] > 0.f && ik > dikcumul) {
+        for (int ik2 = ik1; ik2 <= ik - dikcumul; ++ik2) {
+          const float c2 = q.sig[ik2] / t.wn1[ik2 - 1];
+          const int is2 = (ik2 - 1) * nth;
+          const float cumulwiso = ((__builtin_fabsf(c2 - cc) * q.dsip[ik2]) / (c2 * 0.5f)) * q.dth;
+          float v = 0.f;
+          for (int i = is2; i < is2 + nth; ++i) v = v + t.X[i];
+          renew = renew - cumulwiso * v;
+        }
+      }
+      if (cs[2] < 0.f && ik > dikcumul) {
+        renew = 0.f;
+        for (int ik2 = ik1; ik2 <= ik - dikcumul; ++ik2) {
+          if (bth0w[ik2 - 1] > q.ssdsbr) {
+            const float* bl = t.X + (ik2 - 1) * nth;
+            const float* kr = q.cumulk + (static_cast<size_t>(ik0) * nk + (ik2 - 1)) * nth;
+            float v = 0.f;
+            int j = 0;
+            for (; j < ith; ++j) v = v + kr[ith - j] * bl[j];
+            for (; j < nth; ++j) v = v + kr[j - ith] * bl[j];
+            renew = renew + v;
+          }
+        }
+      }
+      const float coswind = q.ecos[ith] * cu + q.esin[ith] * su;
+      const float dturb = facturb2 * mx(coswind, 0.f);
+      const float diag2 = (cs[2] * renew + dturb) + dvisc;
+      t.vdds[b] = t.vdds[b] + diag2;
+      t.vsds[b] = t.vsds[b] + t.spec[b] * diag2;
+    }
+  }
+
+  // Whitecap parameters (early return keeps WHITECAP).  COEF4/COEF5 are zero above IKC;
+  // BRM12 is never set by this routine and stays 0.
+  if (!(f.flogrd57 || f.flogrd58)) return;
+  float wc0 = 0.f, wc1 = 0.f, wc3 = 0.f;
+  const int ikc = ffloor(q.aaircmin) < nk ? ffloor(q.aaircmin) : nk;
+  const float tstrfac = (0.8f * 2.f) * kPi, tmaxfac = (5.f * 2.f) * kPi;
+  for (int base = 0; base < nk; base += G) {
+    const int ik = base + l + 1;
+    float coef4 = 0.f, coef5 = 0.f, mft = 0.f;
+    if (ik <= nk) {
+      const float kk = t.wn1[ik - 1];
+      if (ik <= ikc) {
+        const float cc = q.sig[ik] / kk;
+        const int is0 = (ik - 1) * nth;
+        float v = 0.f;
+        for (int i = is0; i < is0 + nth; ++i) v = v + t.X[i];
+        const float cgk = t.cg1[ik - 1];
+        coef4 = ((((((cc * cc) * v) * 2.f) * kPi) / kGrav) * cs[6]) * q.dden[ik - 1] / (q.sig[ik] * cgk);
+        const float pw = libm::powf(0.f / cs[1], 3.f / 5.f);
+        float ww = 0.f;
+        for (int i = is0; i < is0 + nth; ++i) ww = ww + (t.X[i] * (-cs[1])) * pw;
+        coef5 = ((((cc * cc) * cc) * ww) * q.aairgb / kGrav) * q.dden[ik - 1] / (q.sig[ik] * cgk);
+      }
+      if (f.flogrd58 && ik >= ik1) {
+        const float tstr = tstrfac / q.sig[ik];
+        const float tmax = tmaxfac / q.sig[ik];
+        const float dt = tmax / 50.f;
+        for (int it = 1; it <= 50; ++it) {
+          const float tt = static_cast<float>(it) * dt;
+          if (tt < tstr)
+            mft = mft + ((0.4f / (kk * tstr)) * tt) * dt;
+          else
+            mft = mft + ((0.4f / kk) * libm::expf(-((tt - tstr) / 3.8f))) * dt;
+        }
+        mft = mft / tmax;
+      }
+    }
+    const int cnt = nk - base;
+#define ST(k)                                                \
+  if (k < cnt) {                                             \
+    const int ikk = base + k + 1;                            \
+    const float c4 = bc<k>(coef4);                           \
+    const float c5 = bc<k>(coef5);                           \
+    const float mf = bc<k>(mft);                             \
+    if (f.flogrd57 && ikk >= ik1 && ikk <= ikc) {            \
+      wc0 = wc0 + c4 * (1.f - wc0);                          \
+      wc3 = wc3 + c5;                                        \
+    }                                                        \
+    if (f.flogrd58 && ikk >= ik1) wc1 = wc1 + c4 * mf;       \
+  }
+    WW_ROW16(ST)
+#undef ST
+  }
+  if (l == 0) {
+    whitecap[0] = wc0;
+    whitecap[1] = wc1;
+    whitecap[2] = 0.f;
+    whitecap[3] = wc3;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// W3SLN1 -> DIRF, WNF in W (VSLN(IS) = WNF(IK) * DIRF(ITH))
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ void ln1(const Grp& t, const SrceDev& p, const Sc& c) {
+  const int nk = p.nk, nth = p.nth, l = t.l;
+  const float ustar = c.v[S_USTAR], usdir = t.pin[P_U10DIR], fhigh = c.v[S_FHIGH];
+  float cosu = 0.f, sinu = 0.f;
+  libm::sincosf(usdir, &sinu, &cosu);
+  float* dirf = t.W + t.o.dirf;
+  float* wnf = t.W + t.o.wnf;
+  for (int ith = l; ith < nth; ith += G) {
+    const float m = fmax2(0.f, p.ecos[ith] * cosu + p.esin[ith] * sinu);
+    const float m2 = m * m;
+    dirf[ith] = m2 * m2;
+  }
+  const float u2 = ustar * ustar;
+  const float fac = p.slnc1 * (u2 * u2);
+  const float ff1 = p.fspm * kGrav / (28.f * ustar);
+  const float ff2 = p.fshf * fmin2(p.sig[nk - 1], fhigh);
+  const float ffilt = fmin2(2.f * p.sig[nk - 1], fmax2(ff2, ff1));
+  for (int ik = l; ik < nk; ik += G) {
+    const float rfr = p.sig[ik] / ffilt;
+    float w;
+    if (rfr < 0.5f) {
+      w = 0.f;
+    } else {
+      const float r2 = rfr * rfr;
+      const float rm4 = 1.f / (r2 * r2);
+      w = fac / t.wn1[ik] * libm::expf(-rm4);
+    }
+    wnf[ik] = w;
+  }
+}
+
+// W3SBT1 -> CBETA in W and the cut-off NSCUT: D(IS) = CBETA(MAPWN(IS)) for IS <= NSCUT,
+// else 0; S = D*A, except after the early return (NSCUT = -1), where S = +0 on every bin.
+__device__ __forceinline__ void sbt1(const Grp& t, const SrceDev& p, Sc& c) {
+  const int nk = p.nk, nth = p.nth;
+  const float depth = c.v[S_DEPTH];
+  if (depth * t.wn1[0] > 6.f) {
+    c.iv[I_NSCUT] = -1;
+    return;
+  }
+  const float factor = p.sbtc1 / depth;
+  float* cbeta = t.W + t.o.cbeta;
+  int ik = 1;
+  for (; ik <= nk; ++ik) {
+    const float w = t.wn1[ik - 1];
+    if (w * depth > 6.f) break;
+    const float x = t.cg1[ik - 1] * w / p.sig[ik - 1] - 0.5f;
+    if (t.l == 0) cbeta[ik - 1] = factor * (x > 0.f ? x : 0.f);
+  }
+  c.iv[I_NSCUT] = (ik - 1) * nth;
+}
+
+// W3SDB1 -> S_DTDB and VSDB (the early return keeps both).
+__device__ __forceinline__ void sdb1(const Grp& t, const SrceDev& p, Sc& c) {
+  const int nspec = p.nspec;
+  const float emean = c.v[S_EMEAN], fmean = c.v[S_FMEAN], wnmean = c.v[S_WNMEAN];
+  const float depth = c.v[S_DEPTH];
+  const double thr = static_cast<double>(0x1p-126f);
+  if (static_cast<double>(emean) < 0x1p-1022) return;
+  double hm;
+  if (p.fdonly != 0) {
+    hm = static_cast<double>(p.sdbc2) * static_cast<double>(depth);
+  } else {
+    const float dpos = depth > 0.f ? depth : 0.f;
+    const double wnm = static_cast<double>(wnmean);
+    hm = static_cast<double>(p.sdbc2) / wnm * ::tanh(wnm * static_cast<double>(dpos));
+  }
+  const double hrms = __builtin_sqrt(8. * static_cast<double>(emean));
+  double bb, b;
+  if (hm > thr) {
+    bb = hrms * hrms / (hm * hm);
+    b = __builtin_sqrt(bb);
+  } else {
+    bb = 0.;
+    b = 0.;
+  }
+  double q0 = 0.;
+  if (b <= 0.5) {
+    q0 = 0.;
+  } else if (b <= 1.) {
+    const double tt = 2. * b - 1.;
+    q0 = tt * tt;
+  }
+  double qb;
+  if (b <= 0.2) {
+    qb = 0.;
+  } else if (b < 1.) {
+    const double arg = ::exp((q0 - 1.) / bb);
+    qb = q0 - bb * (q0 - arg) / (bb - arg);
+    for (int i = 1; i <= 3; ++i) qb = ::exp((qb - 1.) / bb);
+  } else {
+    qb = static_cast<double>(1.0f - 0x1p-126f);
+  }
+  double cbj;
+  if (b != 1. && bb > thr && __builtin_fabs(bb - qb) > thr)
+    cbj = (bb < 1.0) ? 2. * static_cast<double>(p.sdbc1) * qb * static_cast<double>(fmean) / bb
+                     : 2. * static_cast<double>(p.sdbc1) * static_cast<double>(fmean) * bb;
+  else
+    cbj = 0.;
+  const float dv = static_cast<float>(-cbj);
+  c.v[S_DTDB] = dv;
+  for (int is = t.l; is < nspec; is += G) t.vsdb[is] = dv * t.spec[is];
+}
+
+// ------------------------------------------------------------------------------------
+// Kernels
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void init_kernel(State s, St4Dev q, SrceDev pp, StepFlags f) {
+  extern __shared__ float lds[];
+  for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < kMaxIter;
+       i += static_cast<int>(gridDim.x * blockDim.x))
+    s.count[i] = 0;
+  const int g = static_cast<int>(threadIdx.x) / G;
+  const int p = static_cast<int>(blockIdx.x) * kPoints + g;
+  if (p >= s.npts) return;
+  const int nk = pp.nk, nth = pp.nth, nspec = pp.nspec;
+  const Grp t = make_grp(s, nk, nth, p, lds, g);
+  const int l = t.l;
+  Sc c;
+  const float* psc = s.psc + static_cast<size_t>(p) * N_PSC;
+  for (int i = 0; i < N_PSC; ++i) c.v[i] = psc[i];
+  for (int i = N_PSC; i < N_SCAL; ++i) c.v[i] = 0.f;
+  for (int i = 0; i < N_ISCAL; ++i) c.iv[i] = 0;
+  float* whitecap = s.whitecap + static_cast<size_t>(p) * 4;
+  float* taubbl = s.taubbl + static_cast<size_t>(p) * 2;
+  float* tauice = s.tauice + static_cast<size_t>(p) * 2;
+  for (int b = l; b < nspec; b += G) {
+    const float a = t.spec[b];
+    t.X[b] = a;
+    t.vsdb[b] = 0.f;
+    t.llws[b] = f.it == 0 ? 1 : 0;
+  }
+  if (l == 0) {
+    for (int j = 0; j < 4; ++j) whitecap[j] = 0.f;
+    taubbl[0] = taubbl[1] = 0.f;
+    tauice[0] = tauice[1] = 0.f;
+  }
+  c.v[S_DTDB] = 0.f;
+  const float d_inp = t.pin[P_D_INP], dair = t.pin[P_DAIR];
+  c.v[S_DEPTH] = mx(d_inp, pp.dmin);
+  c.v[S_DRAT] = dair / kDwat;
+  c.v[S_DTDYN] = 0.f;
+  c.v[S_DTTOT] = 0.f;
+  c.v[S_PHIAW] = 0.f;
+  c.v[S_CHARN] = 0.f;
+  c.v[S_TWS] = 0.f;
+  c.v[S_PHIBBL] = 0.f;
+  c.v[S_TAUWIX] = 0.f;
+  c.v[S_TAUWIY] = 0.f;
+  c.v[S_TAUWNX] = 0.f;
+  c.v[S_TAUWNY] = 0.f;
+  c.v[S_TAUWAX] = 0.f;
+  c.v[S_TAUWAY] = 0.f;
+  c.v[S_PHICE] = 0.f;
+  c.v[S_TAUOCX] = 0.f;
+  c.v[S_TAUOCY] = 0.f;
+  c.v[S_WNMEAN] = 0.f;
+  c.v[S_DLWMEAN] = 0.f;
+  wave_sync();
+  const float sintail4 = q.sintailpar[3];
+  if (sintail4 > 0.5f) {
+    c.v[S_TAUWX] = 0.f;
+    c.v[S_TAUWY] = 0.f;
+  }
+  if (f.it == 0) {
+    c.v[S_TAUWX] = 0.f;
+    c.v[S_TAUWY] = 0.f;
+    c.v[S_USTAR] = 0.f;
+    c.v[S_USTDIR] = 0.f;
+  } else {
+    // LLWS is read here before anything sets it; only FMEANWS depends on it and the W3SPR4
+    // below overwrites it (TAUWBUG, the only configuration accepted with IT > 0).
+    spr4(t, q, c);
+    if (sintail4 > 0.5f) sin4(t, q, c);
+  }
+  spr4(t, q, c);
+  c.v[S_TWS] = 1.f / c.v[S_FMEANWS];
+  for (int b = l; b < nspec; b += G) t.specinit[b] = t.X[b];
+  c.v[S_FHIGH] = mx((pp.ffxfm + 0.f) * mx(c.v[S_FMEANWS], c.v[S_FMEAN1]), pp.ffxpm / c.v[S_USTAR]);
+  // the first sub-step's input terms and KDMEAN
+  sin4(t, q, c);
+  if (l == 0) {
+    s.kdmean[p] = c.v[S_WNMEAN] * c.v[S_DEPTH];
+    s.list[0][p] = p;
+  }
+  store_sc(s, t, c);
+}
+
+__global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp, StepFlags f, int iter,
+                                                  int n) {
+  extern __shared__ float lds[];
+  const int g = static_cast<int>(threadIdx.x) / G;
+  const int slot = static_cast<int>(blockIdx.x) * kPoints + g;
+  if (slot >= n) return;
+  const int p = s.list[iter & 1][slot];
+  const int nk = pp.nk, nth = pp.nth, nspec = pp.nspec;
+  const Grp t = make_grp(s, nk, nth, p, lds, g);
+  const int l = t.l;
+  Sc c;
+  load_sc(s, p, c);
+  float* whitecap = s.whitecap + static_cast<size_t>(p) * 4;
+  for (int b = l; b < nspec; b += G) t.X[b] = t.spec[b];
+  c.iv[I_NSTEPS] = c.iv[I_NSTEPS] + 1;
+  if (pp.ln1) ln1(t, pp, c);
+  wave_sync();
+  sds4(t, q, f, c, whitecap);
+  sdb1(t, pp, c);
+  sbt1(t, pp, c);
+  wave_sync();
+
+  const float ice = t.pin[P_ICE];
+  const float isln = mx(mn(1.f - ice * pp.icescales[0], 1.f), 0.f);
+  const float isin = mx(mn(1.f - ice * pp.icescales[1], 1.f), 0.f);
+  const float isnl = mx(mn(1.f - ice * pp.icescales[2], 1.f), 0.f);
+  const float isds = mx(0.f, mn(1.f - ice * pp.icescales[3], 1.f));
+  const bool scale = f.inflags24 != 0;
+  const float dtg = f.dtg;
+  float fhigh = c.v[S_FHIGH];
+  int nkh = mni(nk, fint(pp.facti1 * libm::logf(mx(1.E-7f, fhigh)) + pp.facti2));
+  const int nkh1 = mni(nk, nkh + 1);
+  // NSPECH = NKH1*NTH with the wrap of the compiled reference (NKH = INT_MIN when FHIGH is
+  // +Inf, i.e. USTAR = 0 at U10 = 0, gives NSPECH = NTH for even NTH); never above NSPEC.
+  const int nspech_w = static_cast<int>(static_cast<unsigned>(nkh1) * static_cast<unsigned>(nth));
+  const int nspech = nspech_w < nspec ? nspech_w : nspec;
+  float dttot = c.v[S_DTTOT];
+  float dt = mn(pp.dtmax, dtg - dttot);
+  const float wl = t.wn1[nk - 1];
+  const float damlast = pp.facp / (pp.sig[nk - 1] * (wl * (wl * wl)));
+  const float afilt = mx(damlast, pp.xflt * c.v[S_AMAX]);
+  const int nscut = c.iv[I_NSCUT];
+  const float* dirf = t.W + t.o.dirf;
+  const float* wnf = t.W + t.o.wnf;
+  const float* cbeta = t.W + t.o.cbeta;
+
+  // The time-step limit over the bins below NKH1: each lane's MIN, then the row's.
+  {
+    float dtl = dt;
+    for (int is = l; is < nspech; is += G) {
+      const int ik0 = is / nth;
+      const int ith = is - ik0 * nth;
+      const float w = t.wn1[ik0];
+      const float dam = pp.facp / (pp.sig[ik0] * (w * (w * w)));
+      float vsln = pp.ln1 ? wnf[ik0] * dirf[ith] : 0.f;
+      float vsnl = t.vsnl[is], vdnl = t.vdnl[is], vsin = t.vsin[is], vdin = t.vdin[is];
+      float vsds = t.vsds[is], vdds = t.vdds[is];
+      if (scale) {
+        vsnl = isnl * vsnl;
+        vdnl = isnl * vdnl;
+        vsln = isln * vsln;
+        vsin = isin * vsin;
+        vdin = isin * vdin;
+        vsds = isds * vsds;
+        vdds = isds * vdds;
+      }
+      const float a = t.spec[is];
+      const float vdbt = is < nscut ? cbeta[pp.mapwn[is] - 1] : 0.f;
+      const float vsbt = nscut < 0 ? 0.f : vdbt * a;
+      const float vs = (((vsln + vsin) + vsnl) + vsds) + vsbt;
+      const float vd = ((vdin + vdnl) + vdds) + vdbt;
+      const float damax = mn(mx(pp.xrel * t.specinit[is], afilt), dam);
+      const float afac = 1.f / mx(__builtin_fabsf(vs / damax), 1.E-10f);
+      dtl = mn(dtl, afac / mx(mn(vd, 0.f) * afac + 1.f, 1.E-10f));
+    }
+#define ST(k) dt = mn(dt, bc<k>(dtl));
+    WW_ROW16(ST)
+#undef ST
+  }
+  dt = mx(dt, 0.5f);
+  c.v[S_DTDYN] = c.v[S_DTDYN] + dt;
+  const float rest = dtg - dttot;
+  const int idt = 1 + fint((rest * 0.99f) / dt);
+  dt = rest / static_cast<float>(idt);
+  const bool shave = dt < pp.dtmin && dt < rest;
+  c.iv[I_SHAVE] = shave ? 1 : 0;
+  dt = mx(mn(pp.dtmin, rest), dt);
+  const float hdt = dt;
+  dttot = dttot + dt;
+  c.v[S_DTTOT] = dttot;
+
+  // Update (bins below NKH1), W3SDB1 update, flux sums (all bins), in bin order.
+  const float vddb = c.v[S_DTDB];
+  float phiaw = c.v[S_PHIAW], phibbl = c.v[S_PHIBBL], wc2 = 0.f;
+  for (int b0 = 0; b0 < nspec; b0 += G) {
+    const int is = b0 + l;
+    float tpa = 0.f, tpb = 0.f, twc = 0.f, vsin = 0.f;
+    if (is < nspec) {
+      const int ik0 = is / nth;
+      const int ith = is - ik0 * nth;
+      const float a0 = t.spec[is];
+      const float vdbt = is < nscut ? cbeta[pp.mapwn[is] - 1] : 0.f;
+      const float vsbt = nscut < 0 ? 0.f : vdbt * a0;
+      vsin = t.vsin[is];
+      float vdin = t.vdin[is];
+      float a = a0;
+      if (is < nspech) {
+        const float w = t.wn1[ik0];
+        const float dam = pp.facp / (pp.sig[ik0] * (w * (w * w)));
+        float vsln = pp.ln1 ? wnf[ik0] * dirf[ith] : 0.f;
+        float vsnl = t.vsnl[is], vdnl = t.vdnl[is], vsds = t.vsds[is], vdds = t.vdds[is];
+        if (scale) {
+          vsnl = isnl * vsnl;
+          vdnl = isnl * vdnl;
+          vsln = isln * vsln;
+          vsin = isin * vsin;
+          vdin = isin * vdin;
+          vsds = isds * vsds;
+          vdds = isds * vdds;
+        }
+        const float vs = (((vsln + vsin) + vsnl) + vsds) + vsbt;
+        const float vd = ((vdin + vdnl) + vdds) + vdbt;
+        const float e1 = (dt * vs) / mx(1.f - hdt * vd, 1.f);
+        if (shave) {
+          const float e2 = __builtin_copysignf(mn(__builtin_fabsf(e1), dam), e1);
+          a = mx(a + e2, 0.f);
+        } else {
+          a = mx(a + e1, 0.f);
+        }
+        const float e3 = (dt * t.vsdb[is]) / mx(1.f - hdt * vddb, 1.f);
+        a = mx(a + e3, 0.f);
+        t.spec[is] = a;
+      }
+      t.X[is] = a;
+      const float factor = pp.dden[ik0] / t.cg1[ik0];
+      tpa = ((vsin * dt) * factor) / mx(1.f - vdin * dt, 1.f);
+      tpb = ((vsbt * dt) * factor) / mx(1.f - vdbt * dt, 1.f);
+      twc = a * factor;
+    }
+    const int cnt = nspec - b0;
+#define ST(k)                                   \
+  if (k < cnt) {                                \
+    phiaw = bc<k>(tpa) + phiaw;                 \
+    phibbl = phibbl - bc<k>(tpb);               \
+    const float vk = bc<k>(vsin);               \
+    const float wk = bc<k>(twc);                \
+    if (vk > 0.f) wc2 = wc2 + wk;               \
+  }
+    WW_ROW16(ST)
+#undef ST
+  }
+  wave_sync();
+  c.v[S_PHIAW] = phiaw;
+  c.v[S_PHIBBL] = phibbl;
+  if (l == 0) whitecap[2] = __builtin_sqrtf(wc2) * 4.f;
+  const float drat = c.v[S_DRAT];
+  c.v[S_TAUWIX] = c.v[S_TAUWIX] + (c.v[S_TAUWX] * drat) * dt;
+  c.v[S_TAUWIY] = c.v[S_TAUWIY] + (c.v[S_TAUWY] * drat) * dt;
+  c.v[S_TAUWNX] = c.v[S_TAUWNX] + (drat * c.v[S_TAUWAX]) * dt;
+  c.v[S_TAUWNY] = c.v[S_TAUWNY] + (drat * c.v[S_TAUWAY]) * dt;
+
+  spr4(t, q, c);
+  const float u10abs = t.pin[P_U10ABS];
+  const float fage = pp.ffxfa * libm::tanhf((((u10abs * 0.3f) * c.v[S_FMEANWS]) * kTpi) / kGrav);
+  const float fh1 = (pp.ffxfm + fage) * c.v[S_FMEAN1];
+  const float fh2 = pp.ffxpm / c.v[S_USTAR];
+  fhigh = mn(mx(fh1, fh2), pp.sig[nk - 1]);
+  c.v[S_FHIGH] = fhigh;
+  nkh = mxi(mni(nkh1, fint(pp.facti1 * libm::logf(mx(1.E-7f, fhigh)) + pp.facti2)), 2);
+  if (dttot >= dtg * 0.9999f) {
+    const float wnmean = c.v[S_WNMEAN];
+    const float hm = (pp.fhmax * libm::tanhf(mx(0.f, t.pin[P_D_INP]) * wnmean)) /
+                     mx(wnmean, 9.99999974737875163555145263671875e-5f);
+    const float em = (hm * hm) * 6.25e-2f;
+    const float emean = c.v[S_EMEAN];
+    if (emean > em && emean > 1.E-30f) {
+      for (int is = l; is < nspec; is += G) {
+        const float a = em * (t.X[is] / emean);
+        t.X[is] = a;
+        t.spec[is] = a;
+      }
+      c.v[S_EMEAN] = em;
+    }
+  }
+  wave_sync();
+  for (int ik = nkh; ik < nk; ++ik)
+    for (int ith = l; ith < nth; ith += G) {
+      const float a = t.X[ith + (ik - 1) * nth] * pp.fachfa + 0.f;
+      t.X[ith + ik * nth] = a;
+      t.spec[ith + ik * nth] = a;
+    }
+  wave_sync();
+  sin4(t, q, c);
+  const bool done = dttot >= 0.9999f * dtg;
+  if (q.sintailpar[3] < 0.5f) {
+    spr4(t, q, c);
+    // the next sub-step's input terms see the u* and z0 of this W3SPR4
+    if (!done) sin4(t, q, c);
+  }
+  c.iv[I_DONE] = done ? 1 : 0;
+  if (l == 0) {
+    s.kdmean[p] = c.v[S_WNMEAN] * c.v[S_DEPTH];
+    if (!done) {
+      const int at = atomicAdd(&s.count[iter], 1);
+      s.list[(iter + 1) & 1][at] = p;
+    }
+  }
+  store_sc(s, t, c);
+}
+
+__global__ __launch_bounds__(64) void finish_kernel(State s, St4Dev q, SrceDev pp, StepFlags f) {
+  const int g = static_cast<int>(threadIdx.x) / G;
+  const int p = static_cast<int>(blockIdx.x) * kPoints + g;
+  if (p >= s.npts) return;
+  const int nk = pp.nk, nth = pp.nth, nspec = pp.nspec;
+  const int l = static_cast<int>(threadIdx.x) % G;
+  Sc c;
+  load_sc(s, p, c);
+  const size_t ps = static_cast<size_t>(p) * nspec;
+  float* spec = s.spec + ps;
+  const float* specinit = s.specinit + ps;
+  const float* wn1 = s.wn1 + static_cast<size_t>(p) * nk;
+  const float* cg1 = s.cg1 + static_cast<size_t>(p) * nk;
+  const float* pin = s.pin + static_cast<size_t>(p) * N_PIN;
+  float* taubbl = s.taubbl + static_cast<size_t>(p) * 2;
+  float* tauice = s.tauice + static_cast<size_t>(p) * 2;
+  const float dtg = f.dtg;
+  c.v[S_DTDYN] = c.v[S_DTDYN] / static_cast<float>(mxi(c.iv[I_NSTEPS], 1));
+  c.v[S_FCUT] = c.v[S_FHIGH] * kTpiInv;
+  float efinish = 0.f, mwx = 0.f, mwy = 0.f;
+  for (int base = 0; base < nk; base += G) {
+    const int ik = base + l;
+    float te = 0.f, tx = 0.f, ty = 0.f;
+    if (ik < nk) {
+      float eband = 0.f, a1 = 0.f, b1 = 0.f;
+      for (int ith = 0; ith < nth; ++ith) {
+        const float diff = specinit[ith + ik * nth] - spec[ith + ik * nth];
+        eband = diff + eband;
+        a1 = pp.ecos[ith] * diff + a1;
+        b1 = pp.esin[ith] * diff + b1;
+      }
+      const float cgk = cg1[ik], w = wn1[ik];
+      te = (pp.dden[ik] * eband) / cgk;
+      tx = (((pp.dden[ik] * a1) / cgk) * w) / pp.sig[ik];
+      ty = (w * ((pp.dden[ik] * b1) / cgk)) / pp.sig[ik];
+    }
+    const int cnt = nk - base;
+#define ST(k)                          \
+  if (k < cnt) {                       \
+    efinish = bc<k>(te) + efinish;     \
+    mwx = bc<k>(tx) + mwx;             \
+    mwy = bc<k>(ty) + mwy;             \
+  }
+    WW_ROW16(ST)
+#undef ST
+  }
+  wave_sync();
+  const float tauwix = c.v[S_TAUWIX], tauwiy = c.v[S_TAUWIY];
+  const float tbx = taubbl[0], tby = taubbl[1];
+  const float tauox = ((tauwix + mwx * kGrav) - tbx) / dtg;
+  const float tauoy = ((tauwiy + mwy * kGrav) - tby) / dtg;
+  c.v[S_TAUOX] = tauox;
+  c.v[S_TAUOY] = tauoy;
+  const float twix = tauwix / dtg, twiy = tauwiy / dtg;
+  c.v[S_TAUWIX] = twix;
+  c.v[S_TAUWIY] = twiy;
+  c.v[S_TAUWNX] = c.v[S_TAUWNX] / dtg;
+  c.v[S_TAUWNY] = c.v[S_TAUWNY] / dtg;
+  {
+    float sn, cs;
+    libm::sincosf(c.v[S_USTDIR], &sn, &cs);
+    const float ustar = c.v[S_USTAR], coef = pin[P_COEF], dair = pin[P_DAIR];
+    const float kk = ustar * (ustar * (coef * (dair * coef)));
+    c.v[S_TAUOCX] = kk * cs + (tauox - twix) * kDwat;
+    c.v[S_TAUOCY] = kk * sn + (tauoy - twiy) * kDwat;
+  }
+  constexpr float dg = kDwat * kGrav;
+  const float phiaw = c.v[S_PHIAW], phibbl = c.v[S_PHIBBL];
+  c.v[S_PHIOC] = (((phiaw + efinish) - phibbl) * dg) / dtg;
+  c.v[S_PHIAW] = (phiaw * dg) / dtg;
+  c.v[S_PHIBBL] = (phibbl * dg) / dtg;
+  const float ice = pin[P_ICE];
+  float ti0 = tauice[0], ti1 = tauice[1];
+  if (f.inflags24 && ice > 0.f && !pp.ic_numerics) {
+    // Without ice dispersion ATT = 1: the spectrum is unchanged and the sums are of zeros.
+    float phice = 0.f;
+    ti0 = 0.f;
+    ti1 = 0.f;
+    for (int b0 = 0; b0 < nspec; b0 += G) {
+      const int is = b0 + l;
+      float d = 0.f, d0 = 0.f, d1 = 0.f;
+      if (is < nspec) {
+        const int ik0 = is / nth;
+        const float factor = pp.dden[ik0] / cg1[ik0];
+        const float factor2 = ((factor * kGrav) * wn1[ik0]) / pp.sig[ik0];
+        const float a2 = spec[is];
+        const float a = static_cast<float>(1.0 * static_cast<double>(a2));
+        spec[is] = a;
+        d = (a - a2) * factor;
+        d0 = ((a - a2) * factor2) * pp.ecos[is];
+        d1 = ((a - a2) * factor2) * pp.esin[is];
+      }
+      const int cnt = nspec - b0;
+#define ST(k)                      \
+  if (k < cnt) {                   \
+    phice = phice + bc<k>(d);      \
+    ti0 = ti0 - bc<k>(d0);         \
+    ti1 = ti1 - bc<k>(d1);         \
+  }
+      WW_ROW16(ST)
+#undef ST
+    }
+    c.v[S_PHICE] = (((-1.f * kDwat) * kGrav) * phice) / dtg;
+    ti0 = ti0 / dtg;
+    ti1 = ti1 / dtg;
+  }
+  wave_sync();
+  const int nlast = (nspec % 4 == 3) ? nspec - 1 : nspec;
+  for (int is = l; is < nspec; is += G) {
+    float a = f.it == 0 ? specinit[is] : spec[is];
+    a = is < nlast ? mx(a, 0.f) : mx(0.f, a);
+    spec[is] = a;
+  }
+  if (l == 0) {
+    taubbl[0] = tbx / dtg;
+    taubbl[1] = tby / dtg;
+    tauice[0] = ti0;
+    tauice[1] = ti1;
+    float* psc = s.psc + static_cast<size_t>(p) * N_PSC;
+    for (int i = 0; i < N_PSC; ++i) psc[i] = c.v[i];
+    s.shave[p] = c.iv[I_SHAVE];
+  }
+}
+
+inline dim3 groups(int n) { return dim3(static_cast<unsigned>((n + kPoints - 1) / kPoints)); }
+
+}  // namespace
+
+size_t group_lds_bytes(int nk, int nth) {
+  return static_cast<size_t>(kPoints) * static_cast<size_t>(Lds(nk, nth).stride) * sizeof(float);
+}
+
+size_t state_bytes(int nk, int nth, int npts) {
+  const size_t n = static_cast<size_t>(npts);
+  const size_t nspec = static_cast<size_t>(nk) * nth;
+  const auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  size_t b = 0;
+  b += 8 * al(nspec * n * 4);                       // specinit .. vsdb
+  b += al(nspec * n);                               // llws
+  b += al(static_cast<size_t>(N_SCAL) * n * 4);
+  b += al(static_cast<size_t>(N_ISCAL) * n * 4);
+  b += al(n * 4);                                   // kdmean
+  b += 2 * al(n * 4);                               // lists
+  b += al(static_cast<size_t>(kMaxIter) * 4);       // counts
+  return b;
+}
+
+State carve_state(void* base, int nk, int nth, int npts) {
+  State s{};
+  s.npts = npts;
+  const size_t n = static_cast<size_t>(npts);
+  const size_t nspec = static_cast<size_t>(nk) * nth;
+  const auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  char* cur = static_cast<char*>(base);
+  const auto take = [&](size_t bytes) {
+    char* r = cur;
+    cur += al(bytes);
+    return r;
+  };
+  s.specinit = reinterpret_cast<float*>(take(nspec * n * 4));
+  s.vsin = reinterpret_cast<float*>(take(nspec * n * 4));
+  s.vdin = reinterpret_cast<float*>(take(nspec * n * 4));
+  s.vsnl = reinterpret_cast<float*>(take(nspec * n * 4));
+  s.vdnl = reinterpret_cast<float*>(take(nspec * n * 4));
+  s.vsds = reinterpret_cast<float*>(take(nspec * n * 4));
+  s.vdds = reinterpret_cast<float*>(take(nspec * n * 4));
+  s.vsdb = reinterpret_cast<float*>(take(nspec * n * 4));
+  s.llws = reinterpret_cast<unsigned char*>(take(nspec * n));
+  s.sc = reinterpret_cast<float*>(take(static_cast<size_t>(N_SCAL) * n * 4));
+  s.isc = reinterpret_cast<int*>(take(static_cast<size_t>(N_ISCAL) * n * 4));
+  s.kdmean = reinterpret_cast<float*>(take(n * 4));
+  s.list[0] = reinterpret_cast<int*>(take(n * 4));
+  s.list[1] = reinterpret_cast<int*>(take(n * 4));
+  s.count = reinterpret_cast<int*>(take(static_cast<size_t>(kMaxIter) * 4));
+  return s;
+}
+
+hipError_t k_init(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, hipStream_t st) {
+  const unsigned lds = static_cast<unsigned>(group_lds_bytes(p.nk, p.nth));
+  hipLaunchKernelGGL(init_kernel, groups(s.npts), dim3(64), lds, st, s, q, p, f);
+  return hipGetLastError();
+}
+
+hipError_t k_step(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, int iter, int n,
+                  hipStream_t st) {
+  const unsigned lds = static_cast<unsigned>(group_lds_bytes(p.nk, p.nth));
+  hipLaunchKernelGGL(step_kernel, groups(n), dim3(64), lds, st, s, q, p, f, iter, n);
+  return hipGetLastError();
+}
+
+hipError_t k_finish(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, hipStream_t st) {
+  hipLaunchKernelGGL(finish_kernel, groups(s.npts), dim3(64), 0, st, s, q, p, f);
+  return hipGetLastError();
+}
+
+}  // namespace srce
+}  // namespace ww3gpu

@@ -63,6 +63,9 @@ SIGNATURES = {
     "ww_srce_last_error": (c_int, []),
     "ww3gpu_srce_batch_dev": (c_int, [c_int] * 5 + [c_float, c_int] + [c_void_p] * 10),
     "ww3gpu_srce_substeps": (c_int, [c_int]),
+    "ww3gpu_srce_point_substeps": (c_long, [c_int]),
+    "ww3gpu_srce_timing": (c_int, [c_int, c_int]),
+    "ww3gpu_srce_timers": (c_int, [c_int, c_void_p, c_void_p, c_int]),
     "ww3gpu_insin4_tables": (c_int, [c_void_p, c_int, c_void_p, c_int] + [c_void_p] * 7),
 }
 
