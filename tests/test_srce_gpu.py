@@ -84,6 +84,8 @@ def _variant(nk=25, nth=24, npts=64, seed=7, st4=None, st4i=None, srce=None, src
 VARIANTS = {
     "nk32_nth36": dict(nk=32, nth=36),
     "nk25_nth24": dict(),
+    "nk20_nth12": dict(nk=20, nth=12),
+    "nk36_nth36": dict(nk=36, nth=36),
     "stab3": dict(st4i={"STAB3": 1}, pin={"AS": -1.5}),
     "ig1": dict(st4i={"IG1": 1}, st4={"IGPARS5": 3.0}),
     "sat_isotropic": dict(st4={"SSDSDTH": 180.0}),

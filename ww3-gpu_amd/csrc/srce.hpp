@@ -27,6 +27,9 @@ namespace srce {
 
 constexpr int kGroup = 16;  ///< lanes per point (one DPP row)
 
+/// Row length of the mirrored cumulative table: 2*NTH-1 rounded up to a multiple of 4.
+__host__ __device__ constexpr int kMirStride(int nth) { return (2 * nth - 1 + 3) / 4 * 4; }
+
 /// ST4 scalars (W3GDATMD SRCP and W3SRC4MD module variables) and device table pointers.
 struct St4Dev {
   int nk, nth, nspec, stab3, ig1, nsat;
@@ -46,6 +49,9 @@ struct St4Dev {
   /// CUMULW(IS2,IS) as a function of (IK, IK2, |ITH2-ITH|): cumulk[((IK-1)*NK + IK2-1)*NTH + D]
   /// (INSIN4's table has this structure; the commit checks every entry against it).
   const float* cumulk;
+  /// The same rows mirrored: cumulm[((IK-1)*NK + IK2-1)*kMirStride(NTH) + M] = CUMULW for
+  /// |ITH2-ITH| = |M - (NTH-1)|, M = 0 .. 2*NTH-2, so that ITH2-ITH+NTH-1 indexes it directly.
+  const float* cumulm;
   const float* fwtable; ///< FWTABLE(0:300), or null when SWELLF(2) == 0
 };
 
@@ -104,6 +110,7 @@ struct State {
   float* vsds;            ///< [npts][NSPEC] W3SDS4's S of the current sub-step
   float* vdds;            ///< [npts][NSPEC] W3SDS4's D of the current sub-step
   float* vsdb;            ///< [npts][NSPEC] W3SDB1's S, kept across sub-steps (its early return)
+  float* rnw;             ///< [npts][NSPEC] W3SDS4's cumulative term RENEW of the sub-step
   unsigned char* llws;    ///< [npts][NSPEC]
   float* sc;              ///< [npts][N_SCAL]
   int* isc;               ///< [npts][N_ISCAL]

@@ -271,6 +271,8 @@ int upload(Grid& g) {
   const size_t o_satw = take(static_cast<size_t>(nsat) * nth);
   const size_t o_sati = take(static_cast<size_t>(nsat) * nth);
   const size_t o_cum = take(static_cast<size_t>(nk) * nk * nth);
+  const int ms = srce::kMirStride(nth);
+  const size_t o_mir = take(static_cast<size_t>(nk) * nk * ms);
   std::vector<float> h(off, 0.f);
   auto put = [&](size_t o, const Arr& a, size_t n) { std::memcpy(&h[o], a.f.data(), n * sizeof(float)); };
   put(o_sig, g.a[WW_ST4_A_SIG], nk + 2);
@@ -298,6 +300,9 @@ int upload(Grid& g) {
       const float k = h[o_cum + (static_cast<size_t>(ik) * nk + ik2) * nth + std::abs(ith2 - ith)];
       if (std::memcmp(&v, &k, sizeof v) != 0) return WW_KOKKOS_ERR_UNSUPPORTED;
     }
+  for (size_t r = 0; r < static_cast<size_t>(nk) * nk; ++r)
+    for (int m = 0; m < 2 * nth - 1; ++m)
+      h[o_mir + r * ms + m] = h[o_cum + r * nth + std::abs(m - (nth - 1))];
   if (g.tables.ensure(off * sizeof(float)) != hipSuccess ||
       hipMemcpy(g.tables.p, h.data(), off * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
     return WW_KOKKOS_ERR_RUNTIME;
@@ -356,6 +361,7 @@ int upload(Grid& g) {
   d.satw = b + o_satw;
   d.sati = reinterpret_cast<const int*>(b + o_sati);
   d.cumulk = b + o_cum;
+  d.cumulm = b + o_mir;
   d.fwtable = nullptr;
   return WW_KOKKOS_OK;
 }

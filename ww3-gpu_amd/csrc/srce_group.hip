@@ -98,7 +98,7 @@ __device__ inline void wave_sync() {
 
 /// Per-point LDS layout, in floats: X (NSPEC), then the small arrays.
 struct Lds {
-  int dirf, wnf, cbeta, bth0, sf, tail, small, stride;
+  int dirf, wnf, cbeta, bth0, sf, tail, sat, vb, rniso, small, stride;
   __host__ __device__ Lds(int nk, int nth) {
     dirf = 0;
     wnf = dirf + nth;
@@ -106,7 +106,10 @@ struct Lds {
     bth0 = cbeta + nk;
     sf = bth0 + nk;
     tail = sf + nk;
-    small = tail + kTailArr * (kTailJ + 2);
+    sat = tail + kTailArr * (kTailJ + 2);  // ints: count, then the saturated bands
+    vb = sat + nk + 1;
+    rniso = vb + nk;
+    small = rniso + nk;
     // X and W of the four points start 8 banks apart
     stride = ((nk * nth + small + 31) / 32) * 32 + 8;
   }
@@ -129,6 +132,7 @@ struct Grp {
   float* vsds;
   float* vdds;
   float* vsdb;
+  float* rnw;
   unsigned char* llws;
   Lds o;
 };
@@ -141,7 +145,8 @@ struct Sc {
 
 __device__ inline Grp make_grp(const State& s, int nk, int nth, int p, float* lds, int g) {
   Grp t{p, static_cast<int>(threadIdx.x) % G, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, Lds(nk, nth)};
+        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+        Lds(nk, nth)};
   const size_t nspec = static_cast<size_t>(nk) * nth;
   t.X = lds + static_cast<size_t>(g) * t.o.stride;
   t.W = t.X + nspec;
@@ -158,6 +163,7 @@ __device__ inline Grp make_grp(const State& s, int nk, int nth, int p, float* ld
   t.vsds = s.vsds + ps;
   t.vdds = s.vdds + ps;
   t.vsdb = s.vsdb + ps;
+  t.rnw = s.rnw + ps;
   t.llws = s.llws + ps;
   return t;
 }
@@ -639,10 +645,93 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
 }
 
 // ------------------------------------------------------------------------------------
+// W3SDS4's anisotropic cumulative term (SSDSC(3) < 0) for a compiled NTH.
+//
+// RENEW(IS) = sum over saturated IK2 <= IK - DIKCUMUL, in IK2 order, of
+// V = sum over ITH2 in order of CUMULW(IS2, IS) * BRLAMBDA(IS2).  For one (IK, IK2), the NTH
+// values of V are a Toeplitz product: CUMULW depends on |ITH2 - ITH| only.  Each lane takes
+// output bands IK = base + l and base + 31 - l (the two together hold about the same number
+// of (IK, IK2) pairs for every lane), keeps the mirrored CUMULW row of a pair in registers and
+// runs the NTH x NTH products with every index fixed at compile time: per pair, 2*NTH loads
+// and NTH*NTH multiply-add pairs on NTH independent accumulators.  RENEW goes to the point's
+// RNW array.  X holds BRLAMBDA; W's sat list holds the saturated bands in order.
+// ------------------------------------------------------------------------------------
+template <int NTH>
+__device__ __forceinline__ void cumul_aniso(const Grp& t, const St4Dev& q, int dikcumul) {
+  constexpr int MS = kMirStride(NTH);
+  const int nk = q.nk, l = t.l;
+  const int* sat = reinterpret_cast<const int*>(t.W + t.o.sat);
+  const int nsat = sat[0];
+  for (int base = 0; base < nk; base += 2 * G) {
+    const int ika = base + l, ikb = base + 2 * G - 1 - l;  // 0-based output bands
+    const int lima = ika + 1 - dikcumul, limb = ikb + 1 - dikcumul;
+    int ca = 0, cb = 0;
+    if (ika < nk && lima > 0)
+      while (ca < nsat && sat[1 + ca] <= lima) ++ca;
+    if (ikb < nk && limb > 0)
+      while (cb < nsat && sat[1 + cb] <= limb) ++cb;
+    float rn[NTH];
+#pragma unroll
+    for (int i = 0; i < NTH; ++i) rn[i] = 0.f;
+    const int total = ca + cb;
+    for (int m = 0; m < total; ++m) {
+      if (m == ca) {  // band a is complete
+        if (ika < nk) {
+#pragma unroll
+          for (int i = 0; i < NTH; ++i) t.rnw[ika * NTH + i] = rn[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NTH; ++i) rn[i] = 0.f;
+      }
+      const bool ina = m < ca;
+      const int ik0 = ina ? ika : ikb;
+      const int ik2 = sat[1 + (ina ? m : m - ca)];
+      const float4* kr4 =
+          reinterpret_cast<const float4*>(q.cumulm + (static_cast<size_t>(ik0) * nk + (ik2 - 1)) * MS);
+      float km[MS];
+#pragma unroll
+      for (int d = 0; d < MS / 4; ++d) {
+        const float4 v = kr4[d];
+        km[4 * d] = v.x;
+        km[4 * d + 1] = v.y;
+        km[4 * d + 2] = v.z;
+        km[4 * d + 3] = v.w;
+      }
+      const float* bl = t.X + (ik2 - 1) * NTH;
+      float acc[NTH];
+#pragma unroll
+      for (int i = 0; i < NTH; ++i) acc[i] = 0.f;
+#pragma unroll
+      for (int j = 0; j < NTH; ++j) {
+        const float b = bl[j];
+#pragma unroll
+        for (int i = 0; i < NTH; ++i) acc[i] = acc[i] + km[j - i + NTH - 1] * b;
+      }
+#pragma unroll
+      for (int i = 0; i < NTH; ++i) rn[i] = rn[i] + acc[i];
+    }
+    if (ca == total) {  // no switch: rn is band a's, band b has no pair
+      if (ika < nk) {
+#pragma unroll
+        for (int i = 0; i < NTH; ++i) t.rnw[ika * NTH + i] = rn[i];
+      }
+      if (ikb < nk) {
+#pragma unroll
+        for (int i = 0; i < NTH; ++i) t.rnw[ikb * NTH + i] = 0.f;
+      }
+    } else if (ikb < nk) {
+#pragma unroll
+      for (int i = 0; i < NTH; ++i) t.rnw[ikb * NTH + i] = rn[i];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // W3SDS4 (W3SRCE's call: A = SPEC, K = WN1, USDIR = USTDIR), SDSBCHOICE = 1.  On entry X
 // holds the spectrum; on return X holds BRLAMBDA, VSDS/VDDS hold SRHS/DDIAG, W's bth0
 // the saturation maxima.
 // ------------------------------------------------------------------------------------
+template <int NTH>
 __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFlags& f, Sc& c, float* whitecap) {
   const int nk = q.nk, nth = q.nth, nspec = q.nspec, l = t.l;
   const float* cs = q.ssdsc;
@@ -753,90 +842,45 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
   }
   wave_sync();
 
-  // Pass 3: wave-turbulence interaction and the cumulative effect.
-  if (cs[2] != 0.f || cs[4] != 0.f || cs[20] != 0.f) {
-    float su, cu;
-    libm::sincosf(usdir, &su, &cu);
-    for (int b = l; b < nspec; b += G) {
-      const int ik0 = b / nth;
-      const int ik = ik0 + 1;
-      if (ik < ik1) continue;
-      const int ith = b - ik0 * nth;
-      const float kk = t.wn1[ik0];
-      const float facturb2 = -(((q.sig[ik] * 2.f) * kk) * facturb);
-      const float dvisc = -(((cs[20] * 4.f) * kk) * kk);
-      const float cc = q.sig[ik] / kk;
-      float renew = 0.f;
-      if (cs[2] > 0.f && ik > dikcumul) {
-        for (int ik2 = ik1; ik2 <= ik - dikcumul; ++ik2) {
-          const float c2 = q.sig[ik2] / t.wn1[ik2 - 1];
-          const int is2 = (ik2 - 1) * nth;
-          const float cumulwiso = ((__builtin_fabsf(c2 - cc) * q.dsip[ik2]) / (c2 * 0.5f)) * q.dth;
+  // Whitecap parameters (the reference computes them after the cumulative term, which does
+  // not touch their inputs).  COEF4/COEF5 are zero above IKC; BRM12 is never set by this
+  // routine and stays 0.  Without FLOGRD(5,7|8) WHITECAP keeps its values.
+  if (f.flogrd57 || f.flogrd58) {
+    float wc0 = 0.f, wc1 = 0.f, wc3 = 0.f;
+    const int ikc = ffloor(q.aaircmin) < nk ? ffloor(q.aaircmin) : nk;
+    const float tstrfac = (0.8f * 2.f) * kPi, tmaxfac = (5.f * 2.f) * kPi;
+    for (int base = 0; base < nk; base += G) {
+      const int ik = base + l + 1;
+      float coef4 = 0.f, coef5 = 0.f, mft = 0.f;
+      if (ik <= nk) {
+        const float kk = t.wn1[ik - 1];
+        if (ik <= ikc) {
+          const float cc = q.sig[ik] / kk;
+          const int is0 = (ik - 1) * nth;
           float v = 0.f;
-          for (int i = is2; i < is2 + nth; ++i) v = v + t.X[i];
-          renew = renew - cumulwiso * v;
+          for (int i = is0; i < is0 + nth; ++i) v = v + t.X[i];
+          const float cgk = t.cg1[ik - 1];
+          coef4 = ((((((cc * cc) * v) * 2.f) * kPi) / kGrav) * cs[6]) * q.dden[ik - 1] / (q.sig[ik] * cgk);
+          const float pw = libm::powf(0.f / cs[1], 3.f / 5.f);
+          float ww = 0.f;
+          for (int i = is0; i < is0 + nth; ++i) ww = ww + (t.X[i] * (-cs[1])) * pw;
+          coef5 = ((((cc * cc) * cc) * ww) * q.aairgb / kGrav) * q.dden[ik - 1] / (q.sig[ik] * cgk);
         }
-      }
-      if (cs[2] < 0.f && ik > dikcumul) {
-        renew = 0.f;
-        for (int ik2 = ik1; ik2 <= ik - dikcumul; ++ik2) {
-          if (bth0w[ik2 - 1] > q.ssdsbr) {
-            const float* bl = t.X + (ik2 - 1) * nth;
-            const float* kr = q.cumulk + (static_cast<size_t>(ik0) * nk + (ik2 - 1)) * nth;
-            float v = 0.f;
-            int j = 0;
-            for (; j < ith; ++j) v = v + kr[ith - j] * bl[j];
-            for (; j < nth; ++j) v = v + kr[j - ith] * bl[j];
-            renew = renew + v;
+        if (f.flogrd58 && ik >= ik1) {
+          const float tstr = tstrfac / q.sig[ik];
+          const float tmax = tmaxfac / q.sig[ik];
+          const float dt = tmax / 50.f;
+          for (int it = 1; it <= 50; ++it) {
+            const float tt = static_cast<float>(it) * dt;
+            if (tt < tstr)
+              mft = mft + ((0.4f / (kk * tstr)) * tt) * dt;
+            else
+              mft = mft + ((0.4f / kk) * libm::expf(-((tt - tstr) / 3.8f))) * dt;
           }
+          mft = mft / tmax;
         }
       }
-      const float coswind = q.ecos[ith] * cu + q.esin[ith] * su;
-      const float dturb = facturb2 * mx(coswind, 0.f);
-      const float diag2 = (cs[2] * renew + dturb) + dvisc;
-      t.vdds[b] = t.vdds[b] + diag2;
-      t.vsds[b] = t.vsds[b] + t.spec[b] * diag2;
-    }
-  }
-
-  // Whitecap parameters (early return keeps WHITECAP).  COEF4/COEF5 are zero above IKC;
-  // BRM12 is never set by this routine and stays 0.
-  if (!(f.flogrd57 || f.flogrd58)) return;
-  float wc0 = 0.f, wc1 = 0.f, wc3 = 0.f;
-  const int ikc = ffloor(q.aaircmin) < nk ? ffloor(q.aaircmin) : nk;
-  const float tstrfac = (0.8f * 2.f) * kPi, tmaxfac = (5.f * 2.f) * kPi;
-  for (int base = 0; base < nk; base += G) {
-    const int ik = base + l + 1;
-    float coef4 = 0.f, coef5 = 0.f, mft = 0.f;
-    if (ik <= nk) {
-      const float kk = t.wn1[ik - 1];
-      if (ik <= ikc) {
-        const float cc = q.sig[ik] / kk;
-        const int is0 = (ik - 1) * nth;
-        float v = 0.f;
-        for (int i = is0; i < is0 + nth; ++i) v = v + t.X[i];
-        const float cgk = t.cg1[ik - 1];
-        coef4 = ((((((cc * cc) * v) * 2.f) * kPi) / kGrav) * cs[6]) * q.dden[ik - 1] / (q.sig[ik] * cgk);
-        const float pw = libm::powf(0.f / cs[1], 3.f / 5.f);
-        float ww = 0.f;
-        for (int i = is0; i < is0 + nth; ++i) ww = ww + (t.X[i] * (-cs[1])) * pw;
-        coef5 = ((((cc * cc) * cc) * ww) * q.aairgb / kGrav) * q.dden[ik - 1] / (q.sig[ik] * cgk);
-      }
-      if (f.flogrd58 && ik >= ik1) {
-        const float tstr = tstrfac / q.sig[ik];
-        const float tmax = tmaxfac / q.sig[ik];
-        const float dt = tmax / 50.f;
-        for (int it = 1; it <= 50; ++it) {
-          const float tt = static_cast<float>(it) * dt;
-          if (tt < tstr)
-            mft = mft + ((0.4f / (kk * tstr)) * tt) * dt;
-          else
-            mft = mft + ((0.4f / kk) * libm::expf(-((tt - tstr) / 3.8f))) * dt;
-        }
-        mft = mft / tmax;
-      }
-    }
-    const int cnt = nk - base;
+      const int cnt = nk - base;
 #define ST(k)                                                \
   if (k < cnt) {                                             \
     const int ikk = base + k + 1;                            \
@@ -849,14 +893,92 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
     }                                                        \
     if (f.flogrd58 && ikk >= ik1) wc1 = wc1 + c4 * mf;       \
   }
-    WW_ROW16(ST)
+      WW_ROW16(ST)
 #undef ST
+    }
+    if (l == 0) {
+      whitecap[0] = wc0;
+      whitecap[1] = wc1;
+      whitecap[2] = 0.f;
+      whitecap[3] = wc3;
+    }
   }
-  if (l == 0) {
-    whitecap[0] = wc0;
-    whitecap[1] = wc1;
-    whitecap[2] = 0.f;
-    whitecap[3] = wc3;
+
+  if (!(cs[2] != 0.f || cs[4] != 0.f || cs[20] != 0.f)) return;
+
+  // The cumulative effect.  Isotropic (SSDSC(3) > 0): one sum of BRLAMBDA per band, then
+  // RENEW per output band.  Otherwise RENEW per bin, by the Toeplitz kernel when NTH is one
+  // of the compiled ones.
+  float* rniso = t.W + t.o.rniso;
+  if (cs[2] > 0.f) {
+    float* vb = t.W + t.o.vb;
+    for (int ik0 = l; ik0 < nk; ik0 += G) {
+      float v = 0.f;
+      for (int i = ik0 * nth; i < ik0 * nth + nth; ++i) v = v + t.X[i];
+      vb[ik0] = v;
+    }
+    wave_sync();
+    for (int ik0 = l; ik0 < nk; ik0 += G) {
+      const int ik = ik0 + 1;
+      float renew = 0.f;
+      if (ik >= ik1 && ik > dikcumul) {
+        const float cc = q.sig[ik] / t.wn1[ik0];
+        for (int ik2 = ik1; ik2 <= ik - dikcumul; ++ik2) {
+          const float c2 = q.sig[ik2] / t.wn1[ik2 - 1];
+          const float cumulwiso = ((__builtin_fabsf(c2 - cc) * q.dsip[ik2]) / (c2 * 0.5f)) * q.dth;
+          renew = renew - cumulwiso * vb[ik2 - 1];
+        }
+      }
+      rniso[ik0] = renew;
+    }
+  } else if (cs[2] < 0.f) {
+    int* sat = reinterpret_cast<int*>(t.W + t.o.sat);
+    if (l == 0) {
+      int n = 0;
+      for (int ik2 = ik1; ik2 <= nk; ++ik2)
+        if (bth0w[ik2 - 1] > q.ssdsbr) sat[1 + n++] = ik2;
+      sat[0] = n;
+    }
+    wave_sync();
+    if constexpr (NTH > 0) cumul_aniso<NTH>(t, q, dikcumul);
+  }
+  wave_sync();
+
+  // Wave-turbulence interaction and the cumulative term into DDIAG and SRHS.
+  float su, cu;
+  libm::sincosf(usdir, &su, &cu);
+  for (int b = l; b < nspec; b += G) {
+    const int ik0 = b / nth;
+    const int ik = ik0 + 1;
+    if (ik < ik1) continue;
+    const int ith = b - ik0 * nth;
+    const float kk = t.wn1[ik0];
+    const float facturb2 = -(((q.sig[ik] * 2.f) * kk) * facturb);
+    const float dvisc = -(((cs[20] * 4.f) * kk) * kk);
+    float renew = 0.f;
+    if (cs[2] > 0.f && ik > dikcumul) renew = rniso[ik0];
+    if (cs[2] < 0.f && ik > dikcumul) {
+      if constexpr (NTH > 0) {
+        renew = t.rnw[b];
+      } else {
+        const int* sat = reinterpret_cast<const int*>(t.W + t.o.sat);
+        const int ms = kMirStride(nth);
+        for (int m = 0; m < sat[0]; ++m) {
+          const int ik2 = sat[1 + m];
+          if (ik2 > ik - dikcumul) break;
+          const float* bl = t.X + (ik2 - 1) * nth;
+          const float* kr = q.cumulm + (static_cast<size_t>(ik0) * nk + (ik2 - 1)) * ms + (nth - 1 - ith);
+          float v = 0.f;
+          for (int j = 0; j < nth; ++j) v = v + kr[j] * bl[j];
+          renew = renew + v;
+        }
+      }
+    }
+    const float coswind = q.ecos[ith] * cu + q.esin[ith] * su;
+    const float dturb = facturb2 * mx(coswind, 0.f);
+    const float diag2 = (cs[2] * renew + dturb) + dvisc;
+    t.vdds[b] = t.vdds[b] + diag2;
+    t.vsds[b] = t.vsds[b] + t.spec[b] * diag2;
   }
 }
 
@@ -1051,6 +1173,7 @@ __global__ __launch_bounds__(64) void init_kernel(State s, St4Dev q, SrceDev pp,
   store_sc(s, t, c);
 }
 
+template <int NTH>
 __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp, StepFlags f, int iter,
                                                   int n) {
   extern __shared__ float lds[];
@@ -1068,7 +1191,7 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
   c.iv[I_NSTEPS] = c.iv[I_NSTEPS] + 1;
   if (pp.ln1) ln1(t, pp, c);
   wave_sync();
-  sds4(t, q, f, c, whitecap);
+  sds4<NTH>(t, q, f, c, whitecap);
   sdb1(t, pp, c);
   sbt1(t, pp, c);
   wave_sync();
@@ -1398,7 +1521,7 @@ size_t state_bytes(int nk, int nth, int npts) {
   const size_t nspec = static_cast<size_t>(nk) * nth;
   const auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   size_t b = 0;
-  b += 8 * al(nspec * n * 4);                       // specinit .. vsdb
+  b += 9 * al(nspec * n * 4);                       // specinit .. vsdb, rnw
   b += al(nspec * n);                               // llws
   b += al(static_cast<size_t>(N_SCAL) * n * 4);
   b += al(static_cast<size_t>(N_ISCAL) * n * 4);
@@ -1428,6 +1551,7 @@ State carve_state(void* base, int nk, int nth, int npts) {
   s.vsds = reinterpret_cast<float*>(take(nspec * n * 4));
   s.vdds = reinterpret_cast<float*>(take(nspec * n * 4));
   s.vsdb = reinterpret_cast<float*>(take(nspec * n * 4));
+  s.rnw = reinterpret_cast<float*>(take(nspec * n * 4));
   s.llws = reinterpret_cast<unsigned char*>(take(nspec * n));
   s.sc = reinterpret_cast<float*>(take(static_cast<size_t>(N_SCAL) * n * 4));
   s.isc = reinterpret_cast<int*>(take(static_cast<size_t>(N_ISCAL) * n * 4));
@@ -1447,7 +1571,12 @@ hipError_t k_init(const State& s, const St4Dev& q, const SrceDev& p, const StepF
 hipError_t k_step(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, int iter, int n,
                   hipStream_t st) {
   const unsigned lds = static_cast<unsigned>(group_lds_bytes(p.nk, p.nth));
-  hipLaunchKernelGGL(step_kernel, groups(n), dim3(64), lds, st, s, q, p, f, iter, n);
+  if (p.nth == 36)
+    hipLaunchKernelGGL(step_kernel<36>, groups(n), dim3(64), lds, st, s, q, p, f, iter, n);
+  else if (p.nth == 24)
+    hipLaunchKernelGGL(step_kernel<24>, groups(n), dim3(64), lds, st, s, q, p, f, iter, n);
+  else
+    hipLaunchKernelGGL(step_kernel<0>, groups(n), dim3(64), lds, st, s, q, p, f, iter, n);
   return hipGetLastError();
 }
 
