@@ -464,6 +464,7 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
       for (int base = 0; base < nth; base += G) {
         const int ith = base + l;
         float dv = 0.f, temp2 = 0.f;
+        float pc = 0.f, ps = 0.f;
         if (ith < nth) {
           const int is = ith + (ik - 1) * nth;  // 0-based
           const float ec = q.ecos[is], es = q.esin[is];
@@ -494,22 +495,20 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
           t.llws[is] = static_cast<unsigned char>(lw);
           dst[is] = dv;
           temp2 = const2 * dv * t.X[is];
+          pc = temp2 * ec;
+          ps = temp2 * es;
         }
         const int cnt = nth - base;
-        const int isb = base + (ik - 1) * nth;
+        // SN (DSTAB < 0) or SS gets TEMP2*ECOS, TEMP2*ESIN of each direction, in ITH order
 #define ST(k)                                       \
   if (k < cnt) {                                    \
-    const float t2 = bc<k>(temp2);                  \
-    const float dk = bc<k>(dv);                     \
-    const float eck = q.ecos[isb + k];              \
-    const float esk = q.esin[isb + k];              \
-    if (dk < 0.f) {                                 \
-      snx = snx + t2 * eck;                         \
-      sny = sny + t2 * esk;                         \
-    } else {                                        \
-      ssx = ssx + t2 * eck;                         \
-      ssy = ssy + t2 * esk;                         \
-    }                                               \
+    const float xk = bc<k>(pc);                     \
+    const float yk = bc<k>(ps);                     \
+    const bool neg = bc<k>(dv) < 0.f;               \
+    snx = neg ? snx + xk : snx;                     \
+    sny = neg ? sny + yk : sny;                     \
+    ssx = neg ? ssx : ssx + xk;                     \
+    ssy = neg ? ssy : ssy + yk;                     \
   }
         WW_ROW16(ST)
 #undef ST
@@ -763,7 +762,19 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
           const int* x = q.sati + q.nsat * ith;
           const float* xb = t.X + ik0 * nth - 1;
           float acc = 0.f;
-          for (int j = 0; j < q.nsat; ++j) acc = acc + w[j] * xb[x[j]];
+          constexpr int NSAT = NTH > 0 ? 2 * (NTH / 2 - 1) + 1 : 0;
+          if (NSAT > 0 && q.nsat == NSAT) {
+            // all weights and indices first, then the gathers, then the sum in J order
+            float wr[NSAT > 0 ? NSAT : 1], ar[NSAT > 0 ? NSAT : 1];
+#pragma unroll
+            for (int j = 0; j < NSAT; ++j) wr[j] = w[j];
+#pragma unroll
+            for (int j = 0; j < NSAT; ++j) ar[j] = xb[x[j]];
+#pragma unroll
+            for (int j = 0; j < NSAT; ++j) acc = acc + wr[j] * ar[j];
+          } else {
+            for (int j = 0; j < q.nsat; ++j) acc = acc + w[j] * xb[x[j]];
+          }
           bv = acc * facsat;
           t.vsds[b] = bv;
         }
