@@ -25,7 +25,9 @@
 namespace ww3gpu {
 namespace srce {
 
-constexpr int kGroup = 16;  ///< lanes per point (one DPP row)
+/// Active-point count at or below which a sub-step runs a wavefront per point instead of a
+/// 16-lane row per point.
+constexpr int kWideBelow = 2048;
 
 /// Row length of the mirrored cumulative table: 2*NTH-1 rounded up to a multiple of 4.
 __host__ __device__ constexpr int kMirStride(int nth) { return (2 * nth - 1 + 3) / 4 * 4; }

@@ -47,8 +47,7 @@ namespace ww3gpu {
 namespace srce {
 namespace {
 
-constexpr int G = kGroup;
-constexpr int kPoints = 64 / G;  // points per workgroup (one wavefront)
+
 constexpr int kTailJ = 50;       // W3SIN4's tail integration steps
 constexpr int kTailArr = 7;      // precomputed tail arrays
 
@@ -79,11 +78,17 @@ __device__ inline int ffloor(float x) {
   return static_cast<float>(i) <= x ? i : static_cast<int>(static_cast<unsigned>(i) - 1u);
 }
 
-/// Lane K's value of v, in every lane of its 16-lane row.
-template <int K>
+/// Lane K's value of v, in every lane of its group: a 16-lane DPP row (row_newbcast), or the
+/// whole wavefront (readlane).
+template <int G, int K>
 __device__ inline float bc(float v) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + K, 0xF, 0xF, false));
+  if constexpr (G == 16) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + K, 0xF, 0xF, false));
+  } else {
+    static_assert(G == 64, "groups are a DPP row or a wavefront");
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), K));
+  }
 }
 
 /// Orders the lanes' LDS and global accesses around it for the compiler (the hardware keeps
@@ -95,6 +100,10 @@ __device__ inline void wave_sync() {
 }
 
 #define WW_ROW16(F) F(0) F(1) F(2) F(3) F(4) F(5) F(6) F(7) F(8) F(9) F(10) F(11) F(12) F(13) F(14) F(15)
+#define WW_ROW16B(F, o) F(o + 0) F(o + 1) F(o + 2) F(o + 3) F(o + 4) F(o + 5) F(o + 6) F(o + 7) \
+  F(o + 8) F(o + 9) F(o + 10) F(o + 11) F(o + 12) F(o + 13) F(o + 14) F(o + 15)
+/// F(k) for every lane k of a group (the entries k >= G are discarded at compile time).
+#define WW_ROW(F) WW_ROW16B(F, 0) WW_ROW16B(F, 16) WW_ROW16B(F, 32) WW_ROW16B(F, 48)
 
 /// Per-point LDS layout, in floats: X (NSPEC), then the small arrays.
 struct Lds {
@@ -143,6 +152,7 @@ struct Sc {
   int iv[N_ISCAL];
 };
 
+template <int G>
 __device__ inline Grp make_grp(const State& s, int nk, int nth, int p, float* lds, int g) {
   Grp t{p, static_cast<int>(threadIdx.x) % G, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
         nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
@@ -241,6 +251,7 @@ __device__ __forceinline__ void calc_ustar(const St4Dev& q, float windspeed, flo
 // ------------------------------------------------------------------------------------
 // W3SPR4 (reads the spectrum from X, LLWS from the point's array)
 // ------------------------------------------------------------------------------------
+template <int G>
 __device__ __forceinline__ void spr4(const Grp& t, const St4Dev& q, Sc& c) {
   const int nk = q.nk, nth = q.nth, nspec = q.nspec, l = t.l;
   const float eps = 1.E-7f;
@@ -262,13 +273,13 @@ __device__ __forceinline__ void spr4(const Grp& t, const St4Dev& q, Sc& c) {
       ps = aa * q.esin[b] * f;
     }
     const int cnt = nspec - b0;
-#define ST(k)                                \
+#define ST(k) if constexpr ((k) < G)                                \
   if (k < cnt) {                             \
-    elcs = elcs + bc<k>(pc);                 \
-    elsn = elsn + bc<k>(ps);                 \
-    amax = mx(amax, bc<k>(aa));              \
+    elcs = elcs + bc<G, k>(pc);                 \
+    elsn = elsn + bc<G, k>(ps);                 \
+    amax = mx(amax, bc<G, k>(aa));              \
   }
-    WW_ROW16(ST)
+    WW_ROW(ST)
 #undef ST
   }
   const float dlw = libm::atan2f(elsn, elcs);
@@ -297,16 +308,16 @@ __device__ __forceinline__ void spr4(const Grp& t, const St4Dev& q, Sc& c) {
       tfw = e2 * pws;
     }
     const int cnt = nk - base;
-#define ST(k)                                \
+#define ST(k) if constexpr ((k) < G)                                \
   if (k < cnt) {                             \
-    emean = emean + bc<k>(e);                \
-    fmean = fmean + bc<k>(tf);               \
-    fmean1 = fmean1 + bc<k>(tf1);            \
-    wnm = wnm + bc<k>(twn);                  \
-    emeanws = emeanws + bc<k>(e2);           \
-    fmeanws = fmeanws + bc<k>(tfw);          \
+    emean = emean + bc<G, k>(e);                \
+    fmean = fmean + bc<G, k>(tf);               \
+    fmean1 = fmean1 + bc<G, k>(tf1);            \
+    wnm = wnm + bc<G, k>(twn);                  \
+    emeanws = emeanws + bc<G, k>(e2);           \
+    fmeanws = fmeanws + bc<G, k>(tfw);          \
   }
-    WW_ROW16(ST)
+    WW_ROW(ST)
 #undef ST
     if (base + G >= nk) {
       elast = __shfl(e, nk - 1 - base, G);
@@ -354,6 +365,7 @@ __device__ __forceinline__ void spr4(const Grp& t, const St4Dev& q, Sc& c) {
 // W3SIN4 (W3SRCE's call: A = SPEC, K = WN2, U = U10ABS, USDIR = U10DIR); S, D into VSIN,
 // VDIN; TAUWNX/Y into TAUWAX/Y.  Band mapping for the per-bin arrays.
 // ------------------------------------------------------------------------------------
+template <int G>
 __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
   const int nk = q.nk, nth = q.nth, l = t.l;
   const float u = t.pin[P_U10ABS], usdir = t.pin[P_U10DIR], as = t.pin[P_AS];
@@ -381,12 +393,12 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
       ta = eb * q.dden[ik] / cgk;
     }
     const int cnt = nk - base;
-#define ST(k)                      \
+#define ST(k) if constexpr ((k) < G)                      \
   if (k < cnt) {                   \
-    uorb = uorb + bc<k>(tu);       \
-    aorb = aorb + bc<k>(ta);       \
+    uorb = uorb + bc<G, k>(tu);       \
+    aorb = aorb + bc<G, k>(ta);       \
   }
-    WW_ROW16(ST)
+    WW_ROW(ST)
 #undef ST
   }
   uorb = 2.f * __builtin_sqrtf(uorb);
@@ -500,17 +512,17 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
         }
         const int cnt = nth - base;
         // SN (DSTAB < 0) or SS gets TEMP2*ECOS, TEMP2*ESIN of each direction, in ITH order
-#define ST(k)                                       \
+#define ST(k) if constexpr ((k) < G)                                       \
   if (k < cnt) {                                    \
-    const float xk = bc<k>(pc);                     \
-    const float yk = bc<k>(ps);                     \
-    const bool neg = bc<k>(dv) < 0.f;               \
+    const float xk = bc<G, k>(pc);                     \
+    const float yk = bc<G, k>(ps);                     \
+    const bool neg = bc<G, k>(dv) < 0.f;               \
     snx = neg ? snx + xk : snx;                     \
     sny = neg ? sny + yk : sny;                     \
     ssx = neg ? ssx : ssx + xk;                     \
     ssy = neg ? ssy : ssy + yk;                     \
   }
-        WW_ROW16(ST)
+        WW_ROW(ST)
 #undef ST
       }
     }
@@ -559,9 +571,9 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
       term = t.X[is] * (cp * (cp * cp));
     }
     const int cnt = nth - base;
-#define ST(k) \
-  if (k < cnt) temp = temp + bc<k>(term);
-    WW_ROW16(ST)
+#define ST(k) if constexpr ((k) < G) \
+  if (k < cnt) temp = temp + bc<G, k>(term);
+    WW_ROW(ST)
 #undef ST
   }
   const float levtail0 = const0 * temp;
@@ -655,7 +667,7 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
 // and NTH*NTH multiply-add pairs on NTH independent accumulators.  RENEW goes to the point's
 // RNW array.  X holds BRLAMBDA; W's sat list holds the saturated bands in order.
 // ------------------------------------------------------------------------------------
-template <int NTH>
+template <int G, int NTH>
 __device__ __forceinline__ void cumul_aniso(const Grp& t, const St4Dev& q, int dikcumul) {
   constexpr int MS = kMirStride(NTH);
   const int nk = q.nk, l = t.l;
@@ -730,7 +742,7 @@ __device__ __forceinline__ void cumul_aniso(const Grp& t, const St4Dev& q, int d
 // holds the spectrum; on return X holds BRLAMBDA, VSDS/VDDS hold SRHS/DDIAG, W's bth0
 // the saturation maxima.
 // ------------------------------------------------------------------------------------
-template <int NTH>
+template <int G, int NTH>
 __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFlags& f, Sc& c, float* whitecap) {
   const int nk = q.nk, nth = q.nth, nspec = q.nspec, l = t.l;
   const float* cs = q.ssdsc;
@@ -744,114 +756,251 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
   const bool iso = q.ssdsdth >= 180.f;
   float* bth0w = t.W + t.o.bth0;
 
-  // Pass 1: B per bin (kept in VSDS), and per band the sum (isotropic case) or MAXVAL.
-  float lim = -__builtin_inff(), asum = 0.f;
-  bool found = false;
-  for (int b0 = 0; b0 < nspec; b0 += G) {
-    const int b = b0 + l;
-    float av = 0.f, bv = 0.f;
-    if (b < nspec) {
-      const int ik0 = b / nth;
-      if (ik0 + 1 >= ik1) {
-        av = t.X[b];
-        if (!iso) {
-          const int ith = b - ik0 * nth;
-          const float kk = t.wn1[ik0];
-          const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;
-          const float* w = q.satw + q.nsat * ith;
-          const int* x = q.sati + q.nsat * ith;
-          const float* xb = t.X + ik0 * nth - 1;
-          float acc = 0.f;
-          constexpr int NSAT = NTH > 0 ? 2 * (NTH / 2 - 1) + 1 : 0;
-          if (NSAT > 0 && q.nsat == NSAT) {
-            // all weights and indices first, then the gathers, then the sum in J order
-            float wr[NSAT > 0 ? NSAT : 1], ar[NSAT > 0 ? NSAT : 1];
+  if constexpr (NTH > 0) {
+    // Saturation, MAXVAL (or the isotropic sum), DDIAG, PB -> BRLAMBDA (into X) and SRHS, band
+    // by band, the lanes over directions (ITH = pass*G + l).  MAXVAL is an exact parallel
+    // reduction: the first maximum in ITH order wins a tie, NaN values are skipped, NaN when
+    // there is no other value, which is what the in-order scan of the reference returns.
+    constexpr int NSAT = 2 * (NTH / 2 - 1) + 1;
+    constexpr int NP = (NTH + G - 1) / G;
+    const bool fixed = q.nsat == NSAT;
+    // One direction per lane (a wavefront per point): its weights and indices for every band.
+    constexpr bool HOIST = NP == 1;
+    float wh[HOIST ? NSAT : 1];
+    int xh[HOIST ? NSAT : 1];
+    if constexpr (HOIST) {
+      if (fixed && l < NTH) {
 #pragma unroll
-            for (int j = 0; j < NSAT; ++j) wr[j] = w[j];
-#pragma unroll
-            for (int j = 0; j < NSAT; ++j) ar[j] = xb[x[j]];
-#pragma unroll
-            for (int j = 0; j < NSAT; ++j) acc = acc + wr[j] * ar[j];
-          } else {
-            for (int j = 0; j < q.nsat; ++j) acc = acc + w[j] * xb[x[j]];
-          }
-          bv = acc * facsat;
-          t.vsds[b] = bv;
+        for (int j = 0; j < NSAT; ++j) {
+          wh[j] = q.satw[q.nsat * l + j];
+          xh[j] = q.sati[q.nsat * l + j];
         }
       }
     }
-    const int cnt = nspec - b0;
-#define ST(k)                                                                   \
-  if (k < cnt) {                                                                \
-    const int bin = b0 + k;                                                     \
-    const int ik0 = bin / nth;                                                  \
-    const int ith = bin - ik0 * nth;                                            \
-    if (ik0 + 1 >= ik1) {                                                       \
-      if (ith == 0) {                                                           \
-        lim = -__builtin_inff();                                                \
-        found = false;                                                          \
-        asum = 0.f;                                                             \
-      }                                                                         \
-      if (iso) {                                                                \
-        asum = asum + bc<k>(av);                                                \
-      } else {                                                                  \
-        const float bk = bc<k>(bv);                                             \
-        if (!found) {                                                           \
-          if (bk >= lim) {                                                      \
-            lim = bk;                                                           \
-            found = true;                                                       \
-          }                                                                     \
-        } else if (bk > lim) {                                                  \
-          lim = bk;                                                             \
-        }                                                                       \
-      }                                                                         \
-      if (ith == nth - 1 && l == 0) {                                           \
-        const float kk = t.wn1[ik0];                                            \
-        const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;       \
-        bth0w[ik0] = iso ? asum * facsat : (found ? lim : __builtin_nanf(""));  \
-      }                                                                         \
-    }                                                                           \
-  }
-    WW_ROW16(ST)
-#undef ST
-  }
-  for (int ik0 = l; ik0 < ik1 - 1 && ik0 < nk; ik0 += G) bth0w[ik0] = 0.f;
-  wave_sync();
-
-  // Pass 2: DDIAG, PB -> BRLAMBDA (into X), SRHS = DDIAG*A.
-  for (int b = l; b < nspec; b += G) {
-    const int ik0 = b / nth;
-    const float a = t.X[b];
-    if (ik0 + 1 < ik1) {
-      t.vdds[b] = 0.f;
-      t.vsds[b] = 0.f * a;  // SRHS = DDIAG * A with DDIAG = 0
-      t.X[b] = 0.f;         // BTH = 0 there, so PB = MAX(-EPSR, 0)**2 * 28.16 = 0
-      continue;
+    for (int ik0 = 0; ik0 < nk; ++ik0) {
+      const int is0 = ik0 * NTH;
+      if (ik0 + 1 < ik1) {
+        for (int ith = l; ith < NTH; ith += G) {
+          const float a = t.X[is0 + ith];
+          t.vdds[is0 + ith] = 0.f;
+          t.vsds[is0 + ith] = 0.f * a;  // SRHS = DDIAG * A with DDIAG = 0
+          t.X[is0 + ith] = 0.f;         // BTH = 0 there, so PB = MAX(-EPSR, 0)**2 * 28.16 = 0
+        }
+        if (l == 0) bth0w[ik0] = 0.f;
+        continue;
+      }
+      const float kk = t.wn1[ik0];
+      const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;
+      float bv[NP], av[NP];
+      float bth0;
+      if (iso) {
+        float asum = 0.f;
+        for (int ith = 0; ith < NTH; ++ith) asum = asum + t.X[is0 + ith];
+        bth0 = asum * facsat;
+#pragma unroll
+        for (int pp = 0; pp < NP; ++pp) {
+          const int ith = pp * G + l;
+          av[pp] = ith < NTH ? t.X[is0 + ith] : 0.f;
+          bv[pp] = bth0;
+        }
+      } else {
+        const float* xb = t.X + is0 - 1;
+        float best = 0.f;
+        int bi = 0x7fffffff;  // no candidate yet
+#pragma unroll
+        for (int pp = 0; pp < NP; ++pp) {
+          const int ith = pp * G + l;
+          float b = 0.f;
+          av[pp] = 0.f;
+          if (ith < NTH) {
+            av[pp] = t.X[is0 + ith];
+            float acc = 0.f;
+            if (fixed) {
+              float wr[NSAT], ar[NSAT];
+              if constexpr (HOIST) {
+#pragma unroll
+                for (int j = 0; j < NSAT; ++j) {
+                  wr[j] = wh[j];
+                  ar[j] = xb[xh[j]];
+                }
+              } else {
+                const float* w = q.satw + q.nsat * ith;
+                const int* x = q.sati + q.nsat * ith;
+#pragma unroll
+                for (int j = 0; j < NSAT; ++j) wr[j] = w[j];
+#pragma unroll
+                for (int j = 0; j < NSAT; ++j) ar[j] = xb[x[j]];
+              }
+#pragma unroll
+              for (int j = 0; j < NSAT; ++j) acc = acc + wr[j] * ar[j];
+            } else {
+              const float* w = q.satw + q.nsat * ith;
+              const int* x = q.sati + q.nsat * ith;
+              for (int j = 0; j < q.nsat; ++j) acc = acc + w[j] * xb[x[j]];
+            }
+            b = acc * facsat;
+            if (b == b && bi == 0x7fffffff) {  // passes run in ITH order: a later one wins only if greater
+              best = b;
+              bi = ith;
+            } else if (b == b && b > best) {
+              best = b;
+              bi = ith;
+            }
+          }
+          bv[pp] = b;
+        }
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) {
+          const float ov = __shfl_xor(best, off, G);
+          const int oi = __shfl_xor(bi, off, G);
+          const bool take = oi != 0x7fffffff &&
+                            (bi == 0x7fffffff || ov > best || (ov == best && oi < bi));
+          best = take ? ov : best;
+          bi = take ? oi : bi;
+        }
+        bth0 = bi == 0x7fffffff ? __builtin_nanf("") : best;
+      }
+      wave_sync();  // every lane's gathers of this band are done before X is overwritten
+      float miche;
+      if (q.ssdsbm[0] == 1.f) {
+        miche = 1.f;
+      } else {
+        const float x = libm::tanhf(mn(kk * depth, 10.f));
+        const float tt = x * (q.ssdsbm[1] + x * (q.ssdsbm[2] + x * (q.ssdsbm[3] + x * q.ssdsbm[4])));
+        miche = tt * tt;
+      }
+      const float coef1 = q.ssdsbr * miche;
+      const float m = mx(bth0 / coef1 - 1.f, 0.f);
+      const float sigk = q.sig[ik0 + 1];
+      const float sdiagiso = ((cs[1] * sigk) * cs[5]) * (m * m);
+      const float coef2 = ((cs[1] * sigk) * (1.f - cs[5])) / (coef1 * coef1);
+#pragma unroll
+      for (int pp = 0; pp < NP; ++pp) {
+        const int ith = pp * G + l;
+        if (ith < NTH) {
+          const float bb = bv[pp];
+          const float dd = sdiagiso + coef2 * libm::powf(mx(bb - coef1, 0.f), q.ssdsp);
+          const float mm = mx(__builtin_sqrtf(bb) - epsr, 0.f);
+          const float pb = (mm * mm) * 28.16f;
+          t.vdds[is0 + ith] = dd;
+          t.X[is0 + ith] = pb / twopisq;
+          t.vsds[is0 + ith] = dd * av[pp];
+        }
+      }
+      if (l == 0) bth0w[ik0] = bth0;
     }
-    const float kk = t.wn1[ik0];
-    const float bth0 = bth0w[ik0];
-    float miche;
-    if (q.ssdsbm[0] == 1.f) {
-      miche = 1.f;
-    } else {
-      const float x = libm::tanhf(mn(kk * depth, 10.f));
-      const float tt = x * (q.ssdsbm[1] + x * (q.ssdsbm[2] + x * (q.ssdsbm[3] + x * q.ssdsbm[4])));
-      miche = tt * tt;
+    wave_sync();
+  } else {
+  // Pass 1: B per bin (kept in VSDS), and per band the sum (isotropic case) or MAXVAL.
+    float lim = -__builtin_inff(), asum = 0.f;
+    bool found = false;
+    for (int b0 = 0; b0 < nspec; b0 += G) {
+      const int b = b0 + l;
+      float av = 0.f, bv = 0.f;
+      if (b < nspec) {
+        const int ik0 = b / nth;
+        if (ik0 + 1 >= ik1) {
+          av = t.X[b];
+          if (!iso) {
+            const int ith = b - ik0 * nth;
+            const float kk = t.wn1[ik0];
+            const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;
+            const float* w = q.satw + q.nsat * ith;
+            const int* x = q.sati + q.nsat * ith;
+            const float* xb = t.X + ik0 * nth - 1;
+            float acc = 0.f;
+            constexpr int NSAT = NTH > 0 ? 2 * (NTH / 2 - 1) + 1 : 0;
+            if (NSAT > 0 && q.nsat == NSAT) {
+              // all weights and indices first, then the gathers, then the sum in J order
+              float wr[NSAT > 0 ? NSAT : 1], ar[NSAT > 0 ? NSAT : 1];
+  #pragma unroll
+              for (int j = 0; j < NSAT; ++j) wr[j] = w[j];
+  #pragma unroll
+              for (int j = 0; j < NSAT; ++j) ar[j] = xb[x[j]];
+  #pragma unroll
+              for (int j = 0; j < NSAT; ++j) acc = acc + wr[j] * ar[j];
+            } else {
+              for (int j = 0; j < q.nsat; ++j) acc = acc + w[j] * xb[x[j]];
+            }
+            bv = acc * facsat;
+            t.vsds[b] = bv;
+          }
+        }
+      }
+      const int cnt = nspec - b0;
+  #define ST(k) if constexpr ((k) < G)                                                                   \
+    if (k < cnt) {                                                                \
+      const int bin = b0 + k;                                                     \
+      const int ik0 = bin / nth;                                                  \
+      const int ith = bin - ik0 * nth;                                            \
+      if (ik0 + 1 >= ik1) {                                                       \
+        if (ith == 0) {                                                           \
+          lim = -__builtin_inff();                                                \
+          found = false;                                                          \
+          asum = 0.f;                                                             \
+        }                                                                         \
+        if (iso) {                                                                \
+          asum = asum + bc<G, k>(av);                                                \
+        } else {                                                                  \
+          const float bk = bc<G, k>(bv);                                             \
+          if (!found) {                                                           \
+            if (bk >= lim) {                                                      \
+              lim = bk;                                                           \
+              found = true;                                                       \
+            }                                                                     \
+          } else if (bk > lim) {                                                  \
+            lim = bk;                                                             \
+          }                                                                       \
+        }                                                                         \
+        if (ith == nth - 1 && l == 0) {                                           \
+          const float kk = t.wn1[ik0];                                            \
+          const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;       \
+          bth0w[ik0] = iso ? asum * facsat : (found ? lim : __builtin_nanf(""));  \
+        }                                                                         \
+      }                                                                           \
     }
-    const float coef1 = q.ssdsbr * miche;
-    const float m = mx(bth0 / coef1 - 1.f, 0.f);
-    const float sigk = q.sig[ik0 + 1];
-    const float sdiagiso = ((cs[1] * sigk) * cs[5]) * (m * m);
-    const float coef2 = ((cs[1] * sigk) * (1.f - cs[5])) / (coef1 * coef1);
-    const float bb = iso ? bth0 : t.vsds[b];
-    const float dd = sdiagiso + coef2 * libm::powf(mx(bb - coef1, 0.f), q.ssdsp);
-    const float mm = mx(__builtin_sqrtf(bb) - epsr, 0.f);
-    const float pb = (mm * mm) * 28.16f;
-    t.vdds[b] = dd;
-    t.X[b] = pb / twopisq;
-    t.vsds[b] = dd * a;
+      WW_ROW(ST)
+  #undef ST
+    }
+    for (int ik0 = l; ik0 < ik1 - 1 && ik0 < nk; ik0 += G) bth0w[ik0] = 0.f;
+    wave_sync();
+  
+    // Pass 2: DDIAG, PB -> BRLAMBDA (into X), SRHS = DDIAG*A.
+    for (int b = l; b < nspec; b += G) {
+      const int ik0 = b / nth;
+      const float a = t.X[b];
+      if (ik0 + 1 < ik1) {
+        t.vdds[b] = 0.f;
+        t.vsds[b] = 0.f * a;  // SRHS = DDIAG * A with DDIAG = 0
+        t.X[b] = 0.f;         // BTH = 0 there, so PB = MAX(-EPSR, 0)**2 * 28.16 = 0
+        continue;
+      }
+      const float kk = t.wn1[ik0];
+      const float bth0 = bth0w[ik0];
+      float miche;
+      if (q.ssdsbm[0] == 1.f) {
+        miche = 1.f;
+      } else {
+        const float x = libm::tanhf(mn(kk * depth, 10.f));
+        const float tt = x * (q.ssdsbm[1] + x * (q.ssdsbm[2] + x * (q.ssdsbm[3] + x * q.ssdsbm[4])));
+        miche = tt * tt;
+      }
+      const float coef1 = q.ssdsbr * miche;
+      const float m = mx(bth0 / coef1 - 1.f, 0.f);
+      const float sigk = q.sig[ik0 + 1];
+      const float sdiagiso = ((cs[1] * sigk) * cs[5]) * (m * m);
+      const float coef2 = ((cs[1] * sigk) * (1.f - cs[5])) / (coef1 * coef1);
+      const float bb = iso ? bth0 : t.vsds[b];
+      const float dd = sdiagiso + coef2 * libm::powf(mx(bb - coef1, 0.f), q.ssdsp);
+      const float mm = mx(__builtin_sqrtf(bb) - epsr, 0.f);
+      const float pb = (mm * mm) * 28.16f;
+      t.vdds[b] = dd;
+      t.X[b] = pb / twopisq;
+      t.vsds[b] = dd * a;
+    }
+    wave_sync();
+  
   }
-  wave_sync();
 
   // Whitecap parameters (the reference computes them after the cumulative term, which does
   // not touch their inputs).  COEF4/COEF5 are zero above IKC; BRM12 is never set by this
@@ -892,19 +1041,19 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
         }
       }
       const int cnt = nk - base;
-#define ST(k)                                                \
+#define ST(k) if constexpr ((k) < G)                                                \
   if (k < cnt) {                                             \
     const int ikk = base + k + 1;                            \
-    const float c4 = bc<k>(coef4);                           \
-    const float c5 = bc<k>(coef5);                           \
-    const float mf = bc<k>(mft);                             \
+    const float c4 = bc<G, k>(coef4);                           \
+    const float c5 = bc<G, k>(coef5);                           \
+    const float mf = bc<G, k>(mft);                             \
     if (f.flogrd57 && ikk >= ik1 && ikk <= ikc) {            \
       wc0 = wc0 + c4 * (1.f - wc0);                          \
       wc3 = wc3 + c5;                                        \
     }                                                        \
     if (f.flogrd58 && ikk >= ik1) wc1 = wc1 + c4 * mf;       \
   }
-      WW_ROW16(ST)
+      WW_ROW(ST)
 #undef ST
     }
     if (l == 0) {
@@ -951,7 +1100,7 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
       sat[0] = n;
     }
     wave_sync();
-    if constexpr (NTH > 0) cumul_aniso<NTH>(t, q, dikcumul);
+    if constexpr (NTH > 0) cumul_aniso<G, NTH>(t, q, dikcumul);
   }
   wave_sync();
 
@@ -996,6 +1145,7 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
 // ------------------------------------------------------------------------------------
 // W3SLN1 -> DIRF, WNF in W (VSLN(IS) = WNF(IK) * DIRF(ITH))
 // ------------------------------------------------------------------------------------
+template <int G>
 __device__ __forceinline__ void ln1(const Grp& t, const SrceDev& p, const Sc& c) {
   const int nk = p.nk, nth = p.nth, l = t.l;
   const float ustar = c.v[S_USTAR], usdir = t.pin[P_U10DIR], fhigh = c.v[S_FHIGH];
@@ -1049,6 +1199,7 @@ __device__ __forceinline__ void sbt1(const Grp& t, const SrceDev& p, Sc& c) {
 }
 
 // W3SDB1 -> S_DTDB and VSDB (the early return keeps both).
+template <int G>
 __device__ __forceinline__ void sdb1(const Grp& t, const SrceDev& p, Sc& c) {
   const int nspec = p.nspec;
   const float emean = c.v[S_EMEAN], fmean = c.v[S_FMEAN], wnmean = c.v[S_WNMEAN];
@@ -1103,16 +1254,17 @@ __device__ __forceinline__ void sdb1(const Grp& t, const SrceDev& p, Sc& c) {
 // ------------------------------------------------------------------------------------
 // Kernels
 // ------------------------------------------------------------------------------------
+template <int G>
 __global__ __launch_bounds__(64) void init_kernel(State s, St4Dev q, SrceDev pp, StepFlags f) {
   extern __shared__ float lds[];
   for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < kMaxIter;
        i += static_cast<int>(gridDim.x * blockDim.x))
     s.count[i] = 0;
   const int g = static_cast<int>(threadIdx.x) / G;
-  const int p = static_cast<int>(blockIdx.x) * kPoints + g;
+  const int p = static_cast<int>(blockIdx.x) * (64 / G) + g;
   if (p >= s.npts) return;
   const int nk = pp.nk, nth = pp.nth, nspec = pp.nspec;
-  const Grp t = make_grp(s, nk, nth, p, lds, g);
+  const Grp t = make_grp<G>(s, nk, nth, p, lds, g);
   const int l = t.l;
   Sc c;
   const float* psc = s.psc + static_cast<size_t>(p) * N_PSC;
@@ -1168,15 +1320,15 @@ __global__ __launch_bounds__(64) void init_kernel(State s, St4Dev q, SrceDev pp,
   } else {
     // LLWS is read here before anything sets it; only FMEANWS depends on it and the W3SPR4
     // below overwrites it (TAUWBUG, the only configuration accepted with IT > 0).
-    spr4(t, q, c);
-    if (sintail4 > 0.5f) sin4(t, q, c);
+    spr4<G>(t, q, c);
+    if (sintail4 > 0.5f) sin4<G>(t, q, c);
   }
-  spr4(t, q, c);
+  spr4<G>(t, q, c);
   c.v[S_TWS] = 1.f / c.v[S_FMEANWS];
   for (int b = l; b < nspec; b += G) t.specinit[b] = t.X[b];
   c.v[S_FHIGH] = mx((pp.ffxfm + 0.f) * mx(c.v[S_FMEANWS], c.v[S_FMEAN1]), pp.ffxpm / c.v[S_USTAR]);
   // the first sub-step's input terms and KDMEAN
-  sin4(t, q, c);
+  sin4<G>(t, q, c);
   if (l == 0) {
     s.kdmean[p] = c.v[S_WNMEAN] * c.v[S_DEPTH];
     s.list[0][p] = p;
@@ -1184,26 +1336,26 @@ __global__ __launch_bounds__(64) void init_kernel(State s, St4Dev q, SrceDev pp,
   store_sc(s, t, c);
 }
 
-template <int NTH>
+template <int G, int NTH>
 __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp, StepFlags f, int iter,
                                                   int n) {
   extern __shared__ float lds[];
   const int g = static_cast<int>(threadIdx.x) / G;
-  const int slot = static_cast<int>(blockIdx.x) * kPoints + g;
+  const int slot = static_cast<int>(blockIdx.x) * (64 / G) + g;
   if (slot >= n) return;
   const int p = s.list[iter & 1][slot];
   const int nk = pp.nk, nth = pp.nth, nspec = pp.nspec;
-  const Grp t = make_grp(s, nk, nth, p, lds, g);
+  const Grp t = make_grp<G>(s, nk, nth, p, lds, g);
   const int l = t.l;
   Sc c;
   load_sc(s, p, c);
   float* whitecap = s.whitecap + static_cast<size_t>(p) * 4;
   for (int b = l; b < nspec; b += G) t.X[b] = t.spec[b];
   c.iv[I_NSTEPS] = c.iv[I_NSTEPS] + 1;
-  if (pp.ln1) ln1(t, pp, c);
+  if (pp.ln1) ln1<G>(t, pp, c);
   wave_sync();
-  sds4<NTH>(t, q, f, c, whitecap);
-  sdb1(t, pp, c);
+  sds4<G, NTH>(t, q, f, c, whitecap);
+  sdb1<G>(t, pp, c);
   sbt1(t, pp, c);
   wave_sync();
 
@@ -1260,8 +1412,8 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
       const float afac = 1.f / mx(__builtin_fabsf(vs / damax), 1.E-10f);
       dtl = mn(dtl, afac / mx(mn(vd, 0.f) * afac + 1.f, 1.E-10f));
     }
-#define ST(k) dt = mn(dt, bc<k>(dtl));
-    WW_ROW16(ST)
+#define ST(k) if constexpr ((k) < G) dt = mn(dt, bc<G, k>(dtl));
+    WW_ROW(ST)
 #undef ST
   }
   dt = mx(dt, 0.5f);
@@ -1325,15 +1477,15 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
       twc = a * factor;
     }
     const int cnt = nspec - b0;
-#define ST(k)                                   \
+#define ST(k) if constexpr ((k) < G)                                   \
   if (k < cnt) {                                \
-    phiaw = bc<k>(tpa) + phiaw;                 \
-    phibbl = phibbl - bc<k>(tpb);               \
-    const float vk = bc<k>(vsin);               \
-    const float wk = bc<k>(twc);                \
+    phiaw = bc<G, k>(tpa) + phiaw;                 \
+    phibbl = phibbl - bc<G, k>(tpb);               \
+    const float vk = bc<G, k>(vsin);               \
+    const float wk = bc<G, k>(twc);                \
     if (vk > 0.f) wc2 = wc2 + wk;               \
   }
-    WW_ROW16(ST)
+    WW_ROW(ST)
 #undef ST
   }
   wave_sync();
@@ -1346,7 +1498,7 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
   c.v[S_TAUWNX] = c.v[S_TAUWNX] + (drat * c.v[S_TAUWAX]) * dt;
   c.v[S_TAUWNY] = c.v[S_TAUWNY] + (drat * c.v[S_TAUWAY]) * dt;
 
-  spr4(t, q, c);
+  spr4<G>(t, q, c);
   const float u10abs = t.pin[P_U10ABS];
   const float fage = pp.ffxfa * libm::tanhf((((u10abs * 0.3f) * c.v[S_FMEANWS]) * kTpi) / kGrav);
   const float fh1 = (pp.ffxfm + fage) * c.v[S_FMEAN1];
@@ -1377,12 +1529,12 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
       t.spec[ith + ik * nth] = a;
     }
   wave_sync();
-  sin4(t, q, c);
+  sin4<G>(t, q, c);
   const bool done = dttot >= 0.9999f * dtg;
   if (q.sintailpar[3] < 0.5f) {
-    spr4(t, q, c);
+    spr4<G>(t, q, c);
     // the next sub-step's input terms see the u* and z0 of this W3SPR4
-    if (!done) sin4(t, q, c);
+    if (!done) sin4<G>(t, q, c);
   }
   c.iv[I_DONE] = done ? 1 : 0;
   if (l == 0) {
@@ -1395,9 +1547,10 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
   store_sc(s, t, c);
 }
 
+template <int G>
 __global__ __launch_bounds__(64) void finish_kernel(State s, St4Dev q, SrceDev pp, StepFlags f) {
   const int g = static_cast<int>(threadIdx.x) / G;
-  const int p = static_cast<int>(blockIdx.x) * kPoints + g;
+  const int p = static_cast<int>(blockIdx.x) * (64 / G) + g;
   if (p >= s.npts) return;
   const int nk = pp.nk, nth = pp.nth, nspec = pp.nspec;
   const int l = static_cast<int>(threadIdx.x) % G;
@@ -1432,13 +1585,13 @@ __global__ __launch_bounds__(64) void finish_kernel(State s, St4Dev q, SrceDev p
       ty = (w * ((pp.dden[ik] * b1) / cgk)) / pp.sig[ik];
     }
     const int cnt = nk - base;
-#define ST(k)                          \
+#define ST(k) if constexpr ((k) < G)                          \
   if (k < cnt) {                       \
-    efinish = bc<k>(te) + efinish;     \
-    mwx = bc<k>(tx) + mwx;             \
-    mwy = bc<k>(ty) + mwy;             \
+    efinish = bc<G, k>(te) + efinish;     \
+    mwx = bc<G, k>(tx) + mwx;             \
+    mwy = bc<G, k>(ty) + mwy;             \
   }
-    WW_ROW16(ST)
+    WW_ROW(ST)
 #undef ST
   }
   wave_sync();
@@ -1488,13 +1641,13 @@ __global__ __launch_bounds__(64) void finish_kernel(State s, St4Dev q, SrceDev p
         d1 = ((a - a2) * factor2) * pp.esin[is];
       }
       const int cnt = nspec - b0;
-#define ST(k)                      \
+#define ST(k) if constexpr ((k) < G)                      \
   if (k < cnt) {                   \
-    phice = phice + bc<k>(d);      \
-    ti0 = ti0 - bc<k>(d0);         \
-    ti1 = ti1 - bc<k>(d1);         \
+    phice = phice + bc<G, k>(d);      \
+    ti0 = ti0 - bc<G, k>(d0);         \
+    ti1 = ti1 - bc<G, k>(d1);         \
   }
-      WW_ROW16(ST)
+      WW_ROW(ST)
 #undef ST
     }
     c.v[S_PHICE] = (((-1.f * kDwat) * kGrav) * phice) / dtg;
@@ -1519,13 +1672,17 @@ __global__ __launch_bounds__(64) void finish_kernel(State s, St4Dev q, SrceDev p
   }
 }
 
-inline dim3 groups(int n) { return dim3(static_cast<unsigned>((n + kPoints - 1) / kPoints)); }
+template <int G>
+inline dim3 groups(int n) { return dim3(static_cast<unsigned>((n + 64 / G - 1) / (64 / G))); }
+
+template <int G>
+inline unsigned lds_of(int nk, int nth) {
+  return static_cast<unsigned>((64 / G) * static_cast<size_t>(Lds(nk, nth).stride) * sizeof(float));
+}
 
 }  // namespace
 
-size_t group_lds_bytes(int nk, int nth) {
-  return static_cast<size_t>(kPoints) * static_cast<size_t>(Lds(nk, nth).stride) * sizeof(float);
-}
+size_t group_lds_bytes(int nk, int nth) { return lds_of<16>(nk, nth); }
 
 size_t state_bytes(int nk, int nth, int npts) {
   const size_t n = static_cast<size_t>(npts);
@@ -1574,25 +1731,32 @@ State carve_state(void* base, int nk, int nth, int npts) {
 }
 
 hipError_t k_init(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, hipStream_t st) {
-  const unsigned lds = static_cast<unsigned>(group_lds_bytes(p.nk, p.nth));
-  hipLaunchKernelGGL(init_kernel, groups(s.npts), dim3(64), lds, st, s, q, p, f);
+  hipLaunchKernelGGL(init_kernel<16>, groups<16>(s.npts), dim3(64), lds_of<16>(p.nk, p.nth), st, s, q, p, f);
+  return hipGetLastError();
+}
+
+template <int G>
+hipError_t launch_step(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, int iter, int n,
+                       hipStream_t st) {
+  const unsigned lds = lds_of<G>(p.nk, p.nth);
+  if (p.nth == 36)
+    hipLaunchKernelGGL((step_kernel<G, 36>), groups<G>(n), dim3(64), lds, st, s, q, p, f, iter, n);
+  else if (p.nth == 24)
+    hipLaunchKernelGGL((step_kernel<G, 24>), groups<G>(n), dim3(64), lds, st, s, q, p, f, iter, n);
+  else
+    hipLaunchKernelGGL((step_kernel<G, 0>), groups<G>(n), dim3(64), lds, st, s, q, p, f, iter, n);
   return hipGetLastError();
 }
 
 hipError_t k_step(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, int iter, int n,
                   hipStream_t st) {
-  const unsigned lds = static_cast<unsigned>(group_lds_bytes(p.nk, p.nth));
-  if (p.nth == 36)
-    hipLaunchKernelGGL(step_kernel<36>, groups(n), dim3(64), lds, st, s, q, p, f, iter, n);
-  else if (p.nth == 24)
-    hipLaunchKernelGGL(step_kernel<24>, groups(n), dim3(64), lds, st, s, q, p, f, iter, n);
-  else
-    hipLaunchKernelGGL(step_kernel<0>, groups(n), dim3(64), lds, st, s, q, p, f, iter, n);
-  return hipGetLastError();
+  // Many points: a 16-lane row per point (four per wavefront) for throughput.  Few points
+  // (the late sub-steps): a wavefront per point, which shortens each point's sub-step.
+  return n > kWideBelow ? launch_step<16>(s, q, p, f, iter, n, st) : launch_step<64>(s, q, p, f, iter, n, st);
 }
 
 hipError_t k_finish(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, hipStream_t st) {
-  hipLaunchKernelGGL(finish_kernel, groups(s.npts), dim3(64), 0, st, s, q, p, f);
+  hipLaunchKernelGGL(finish_kernel<16>, groups<16>(s.npts), dim3(64), 0, st, s, q, p, f);
   return hipGetLastError();
 }
 
