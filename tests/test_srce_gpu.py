@@ -103,6 +103,8 @@ VARIANTS = {
     "shallow": dict(depth_range=(2.5, 12.0), seed=11),
     "dtg_900": dict(flags={"dtg": 900.0}, srce={"DTMAX": 900.0}),
     "swell_turb": dict(st4={("SSWELLF", 2): -0.01, ("SSWELLF", 6): 0.0}),
+    # SINTAILPAR(3) > 0: the tail's Phillips-level terms (seven per-step arrays instead of three)
+    "tail_level": dict(nk=32, nth=36, st4={("SINTAILPAR", 2): 0.04}),
 }
 
 

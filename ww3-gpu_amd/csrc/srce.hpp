@@ -114,6 +114,7 @@ struct State {
   float* vsdb;            ///< [npts][NSPEC] W3SDB1's S, kept across sub-steps (its early return)
   float* rnw;             ///< [npts][NSPEC] W3SDS4's cumulative term RENEW of the sub-step
   unsigned char* llws;    ///< [npts][NSPEC]
+  float* mft;             ///< [npts][NK] the whitecap's MFT integral (fixed for the call)
   float* sc;              ///< [npts][N_SCAL]
   int* isc;               ///< [npts][N_ISCAL]
   float* kdmean;          ///< [npts] the DIA's KDMEAN of the next sub-step

@@ -49,7 +49,7 @@ namespace {
 
 
 constexpr int kTailJ = 50;       // W3SIN4's tail integration steps
-constexpr int kTailArr = 7;      // precomputed tail arrays
+constexpr int kTailStride = kTailJ + 2;  // row length of its per-step arrays in LDS
 
 constexpr float kGrav = 9.806f;
 constexpr float kDwat = 1000.f;
@@ -105,20 +105,26 @@ __device__ inline void wave_sync() {
 /// F(k) for every lane k of a group (the entries k >= G are discarded at compile time).
 #define WW_ROW(F) WW_ROW16B(F, 0) WW_ROW16B(F, 16) WW_ROW16B(F, 32) WW_ROW16B(F, 48)
 
-/// Per-point LDS layout, in floats: X (NSPEC), then the small arrays.
+/// Per-point LDS layout, in floats: X (NSPEC), then the arrays that live through a sub-step
+/// (DIRF, WNF, CBETA), then one region that W3SDS4 (BTH0, the saturated-band list, the
+/// isotropic band sums and RENEW), W3SPR4 (SF) and W3SIN4 (four factors per band during the
+/// stress scan, then the tail integral's per-step terms: 3 arrays, 7 with SINTAILPAR(3) > 0)
+/// use in turn.  5.5 KB per point at NK=32, NTH=36: seven 4-point workgroups per CU.
 struct Lds {
-  int dirf, wnf, cbeta, bth0, sf, tail, sat, vb, rniso, small, stride;
-  __host__ __device__ Lds(int nk, int nth) {
+  int dirf, wnf, cbeta, bth0, sf, pk, sat, vb, rniso, small, stride;
+  __host__ __device__ Lds(int nk, int nth, bool tail7) {
     dirf = 0;
     wnf = dirf + nth;
     cbeta = wnf + nk;
-    bth0 = cbeta + nk;
-    sf = bth0 + nk;
-    tail = sf + nk;
-    sat = tail + kTailArr * (kTailJ + 2);  // ints: count, then the saturated bands
+    const int un = cbeta + nk;
+    bth0 = un;
+    sat = bth0 + nk;  // ints: count, then the saturated bands
     vb = sat + nk + 1;
     rniso = vb + nk;
-    small = rniso + nk;
+    sf = un;
+    pk = un;
+    const int tail = (tail7 ? 7 : 3) * kTailStride;
+    small = un + (tail > 4 * nk + 1 ? tail : 4 * nk + 1);
     // X and W of the four points start 8 banks apart
     stride = ((nk * nth + small + 31) / 32) * 32 + 8;
   }
@@ -144,6 +150,7 @@ struct Grp {
   float* rnw;
   unsigned char* llws;
   Lds o;
+  const float* mft;
 };
 
 /// The point's scalars, in registers while a kernel runs.
@@ -153,10 +160,11 @@ struct Sc {
 };
 
 template <int G>
-__device__ inline Grp make_grp(const State& s, int nk, int nth, int p, float* lds, int g) {
+__device__ inline Grp make_grp(const State& s, const St4Dev& q, int p, float* lds, int g) {
+  const int nk = q.nk, nth = q.nth;
   Grp t{p, static_cast<int>(threadIdx.x) % G, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
         nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-        Lds(nk, nth)};
+        Lds(nk, nth, q.sintailpar[2] > 0.f)};
   const size_t nspec = static_cast<size_t>(nk) * nth;
   t.X = lds + static_cast<size_t>(g) * t.o.stride;
   t.W = t.X + nspec;
@@ -175,6 +183,7 @@ __device__ inline Grp make_grp(const State& s, int nk, int nth, int p, float* ld
   t.vsdb = s.vsdb + ps;
   t.rnw = s.rnw + ps;
   t.llws = s.llws + ps;
+  t.mft = s.mft + static_cast<size_t>(p) * nk;
   return t;
 }
 
@@ -365,9 +374,10 @@ __device__ __forceinline__ void spr4(const Grp& t, const St4Dev& q, Sc& c) {
 // W3SIN4 (W3SRCE's call: A = SPEC, K = WN2, U = U10ABS, USDIR = U10DIR); S, D into VSIN,
 // VDIN; TAUWNX/Y into TAUWAX/Y.  Band mapping for the per-bin arrays.
 // ------------------------------------------------------------------------------------
-template <int G>
+template <int G, int NTH = 0>
 __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
-  const int nk = q.nk, nth = q.nth, l = t.l;
+  const int nk = q.nk, l = t.l;
+  const int nth = NTH > 0 ? NTH : q.nth;  // a compile-time loop bound for the compiled NTH
   const float u = t.pin[P_U10ABS], usdir = t.pin[P_U10DIR], as = t.pin[P_AS];
   const float ustar = c.v[S_USTAR], drat = c.v[S_DRAT], z0 = c.v[S_Z0];
   constexpr float kappa2 = kKappa * kKappa;
@@ -442,6 +452,20 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
     ustarsigma = (1.0f + u / (10.f + u)) * usigma;
   }
 
+  // The scan's per-band factors that do not depend on the running stress and cost a
+  // division, a square root or a logarithm, one band per lane: CM, CONST2, ZCN and the
+  // viscous swell factor (st4_sin4.hpp:211-219).
+  float* pk = t.W + t.o.pk;
+  wave_sync();
+  for (int ik0 = l; ik0 < nk; ik0 += G) {
+    const float kis = t.wn1[ik0];
+    const float s2 = q.sig2[ik0 * nth];
+    pk[ik0] = kis / s2;
+    pk[nk + ik0] = q.dden2[ik0 * nth] / t.cg1[ik0] * kGrav / (q.sig[ik0 + 1] / kis * drat);
+    pk[2 * nk + ik0] = libm::logf(kis * z0);
+    pk[3 * nk + ik0] = -(q.sswellf[4] * drat * 2.f * kis * __builtin_sqrtf(twonu * s2));
+  }
+  wave_sync();
   // The stress scan.  DSTAB(1, IS) (STAB3) is kept in VSIN until the average is formed.
   float ust = ustar, taux = 0.f, tauy = 0.f, cosu = 0.f, sinu = 0.f, ustp = 0.f, usdirp = 0.f;
   float s1x = 0.f, s1y = 0.f, n1x = 0.f, n1y = 0.f;  // istab 1
@@ -464,15 +488,15 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
       usdirp = libm::atan2f(taupy, taupx);
       libm::sincosf(usdirp, &sinu, &cosu);
       const int is1 = 1 + (ik - 1) * nth;
-      const float kis = t.wn1[ik - 1];
       const float s2 = q.sig2[is1 - 1];
-      const float cm = kis / s2;
+      const float cm = pk[ik - 1];
       const float ucn = ustp * cm + q.zzalp;
-      const float const2 = q.dden2[is1 - 1] / t.cg1[ik - 1] * kGrav / (q.sig[ik] / kis * drat);
+      const float const2 = pk[nk + ik - 1];
       const float cnst = s2 * const0;
-      const float zcn = libm::logf(kis * z0);
-      const float swv = -(q.sswellf[4] * drat * 2.f * kis * __builtin_sqrtf(twonu * s2));
+      const float zcn = pk[2 * nk + ik - 1];
+      const float swv = pk[3 * nk + ik - 1];
       const float swt = -(drat * q.sswellf[0] * 16.f * (s2 * s2) / kGrav);
+#pragma unroll 1
       for (int base = 0; base < nth; base += G) {
         const int ith = base + l;
         float dv = 0.f, temp2 = 0.f;
@@ -511,16 +535,18 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
           ps = temp2 * es;
         }
         const int cnt = nth - base;
-        // SN (DSTAB < 0) or SS gets TEMP2*ECOS, TEMP2*ESIN of each direction, in ITH order
+        // SN (DSTAB < 0) or SS gets TEMP2*ECOS, TEMP2*ESIN of each direction, in ITH order.
+        // The sum that a direction does not go to adds -0, which leaves every value unchanged
+        // (x + -0 == x for all x, zeros and NaN included), so each step is one add per sum.
+        const bool neg = dv < 0.f;
+        const float pcn = neg ? pc : -0.f, psn = neg ? ps : -0.f;
+        const float pcs = neg ? -0.f : pc, pss = neg ? -0.f : ps;
 #define ST(k) if constexpr ((k) < G)                                       \
   if (k < cnt) {                                    \
-    const float xk = bc<G, k>(pc);                     \
-    const float yk = bc<G, k>(ps);                     \
-    const bool neg = bc<G, k>(dv) < 0.f;               \
-    snx = neg ? snx + xk : snx;                     \
-    sny = neg ? sny + yk : sny;                     \
-    ssx = neg ? ssx : ssx + xk;                     \
-    ssy = neg ? ssy : ssy + yk;                     \
+    snx = snx + bc<G, k>(pcn);                      \
+    sny = sny + bc<G, k>(psn);                      \
+    ssx = ssx + bc<G, k>(pcs);                      \
+    ssy = ssy + bc<G, k>(pss);                      \
   }
         WW_ROW(ST)
 #undef ST
@@ -582,8 +608,9 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
   const float zinf = libm::logf(sg * sg / kGrav);
   const float dely = (logtpi005 - zinf) / 50.f;
   const float sp2 = q.sintailpar[2];
-  constexpr int TS = kTailJ + 2;
-  float* tl = t.W + t.o.tail;  // cm, cgtail, g0, c1, fm, a0, a1
+  constexpr int TS = kTailStride;
+  float* tl = t.W + t.o.pk;  // cm, cgtail, g0, c1, fm, a0, a1 (over the scan's factors)
+  wave_sync();
   for (int j0 = l; j0 < kTailJ; j0 += G) {
     const int j = j0 + 1;
     const float y = libm::expf(zinf + static_cast<float>(j - 1) * dely);
@@ -735,6 +762,24 @@ __device__ __forceinline__ void cumul_aniso(const Grp& t, const St4Dev& q, int d
       for (int i = 0; i < NTH; ++i) t.rnw[ikb * NTH + i] = rn[i];
     }
   }
+}
+
+// The whitecap's MFT for one band: the 50-step integral of the foam decay over the
+// band's period (st4_sds4.hpp:690-707); it depends on K and SIG only, so a call computes it
+// once per point instead of every sub-step.
+__device__ __forceinline__ float mft_integral(float kk, float sigk) {
+  const float tstr = ((0.8f * 2.f) * kPi) / sigk;
+  const float tmax = ((5.f * 2.f) * kPi) / sigk;
+  const float dt = tmax / 50.f;
+  float mft = 0.f;
+  for (int it = 1; it <= 50; ++it) {
+    const float tt = static_cast<float>(it) * dt;
+    if (tt < tstr)
+      mft = mft + ((0.4f / (kk * tstr)) * tt) * dt;
+    else
+      mft = mft + ((0.4f / kk) * libm::expf(-((tt - tstr) / 3.8f))) * dt;
+  }
+  return mft / tmax;
 }
 
 // ------------------------------------------------------------------------------------
@@ -1008,7 +1053,6 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
   if (f.flogrd57 || f.flogrd58) {
     float wc0 = 0.f, wc1 = 0.f, wc3 = 0.f;
     const int ikc = ffloor(q.aaircmin) < nk ? ffloor(q.aaircmin) : nk;
-    const float tstrfac = (0.8f * 2.f) * kPi, tmaxfac = (5.f * 2.f) * kPi;
     for (int base = 0; base < nk; base += G) {
       const int ik = base + l + 1;
       float coef4 = 0.f, coef5 = 0.f, mft = 0.f;
@@ -1026,19 +1070,7 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
           for (int i = is0; i < is0 + nth; ++i) ww = ww + (t.X[i] * (-cs[1])) * pw;
           coef5 = ((((cc * cc) * cc) * ww) * q.aairgb / kGrav) * q.dden[ik - 1] / (q.sig[ik] * cgk);
         }
-        if (f.flogrd58 && ik >= ik1) {
-          const float tstr = tstrfac / q.sig[ik];
-          const float tmax = tmaxfac / q.sig[ik];
-          const float dt = tmax / 50.f;
-          for (int it = 1; it <= 50; ++it) {
-            const float tt = static_cast<float>(it) * dt;
-            if (tt < tstr)
-              mft = mft + ((0.4f / (kk * tstr)) * tt) * dt;
-            else
-              mft = mft + ((0.4f / kk) * libm::expf(-((tt - tstr) / 3.8f))) * dt;
-          }
-          mft = mft / tmax;
-        }
+        if (f.flogrd58 && ik >= ik1) mft = t.mft[ik - 1];
       }
       const int cnt = nk - base;
 #define ST(k) if constexpr ((k) < G)                                                \
@@ -1264,7 +1296,7 @@ __global__ __launch_bounds__(64) void init_kernel(State s, St4Dev q, SrceDev pp,
   const int p = static_cast<int>(blockIdx.x) * (64 / G) + g;
   if (p >= s.npts) return;
   const int nk = pp.nk, nth = pp.nth, nspec = pp.nspec;
-  const Grp t = make_grp<G>(s, nk, nth, p, lds, g);
+  const Grp t = make_grp<G>(s, q, p, lds, g);
   const int l = t.l;
   Sc c;
   const float* psc = s.psc + static_cast<size_t>(p) * N_PSC;
@@ -1284,6 +1316,10 @@ __global__ __launch_bounds__(64) void init_kernel(State s, St4Dev q, SrceDev pp,
     for (int j = 0; j < 4; ++j) whitecap[j] = 0.f;
     taubbl[0] = taubbl[1] = 0.f;
     tauice[0] = tauice[1] = 0.f;
+  }
+  if (f.flogrd58) {
+    float* mft = s.mft + static_cast<size_t>(p) * nk;
+    for (int ik0 = l; ik0 < nk; ik0 += G) mft[ik0] = mft_integral(t.wn1[ik0], q.sig[ik0 + 1]);
   }
   c.v[S_DTDB] = 0.f;
   const float d_inp = t.pin[P_D_INP], dair = t.pin[P_DAIR];
@@ -1345,7 +1381,7 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
   if (slot >= n) return;
   const int p = s.list[iter & 1][slot];
   const int nk = pp.nk, nth = pp.nth, nspec = pp.nspec;
-  const Grp t = make_grp<G>(s, nk, nth, p, lds, g);
+  const Grp t = make_grp<G>(s, q, p, lds, g);
   const int l = t.l;
   Sc c;
   load_sc(s, p, c);
@@ -1383,17 +1419,40 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
   const float* wnf = t.W + t.o.wnf;
   const float* cbeta = t.W + t.o.cbeta;
 
-  // The time-step limit over the bins below NKH1: each lane's MIN, then the row's.
+  // The time-step limit over the bins below NKH1: each lane's MIN, then the row's.  The
+  // loads of a lane's next bin are issued before the current one is computed.
   {
+    struct Lim {
+      float vsnl, vdnl, vsin, vdin, vsds, vdds, a, ai;
+      int mw;
+    };
+    const auto ld = [&](int is) {
+      Lim v{};
+      if (is < nspech) {
+        v.vsnl = t.vsnl[is];
+        v.vdnl = t.vdnl[is];
+        v.vsin = t.vsin[is];
+        v.vdin = t.vdin[is];
+        v.vsds = t.vsds[is];
+        v.vdds = t.vdds[is];
+        v.a = t.spec[is];
+        v.ai = t.specinit[is];
+        v.mw = is < nscut ? pp.mapwn[is] : 1;
+      }
+      return v;
+    };
     float dtl = dt;
+    Lim nx = ld(l);
     for (int is = l; is < nspech; is += G) {
+      const Lim cu = nx;
+      nx = ld(is + G);
       const int ik0 = is / nth;
       const int ith = is - ik0 * nth;
       const float w = t.wn1[ik0];
       const float dam = pp.facp / (pp.sig[ik0] * (w * (w * w)));
       float vsln = pp.ln1 ? wnf[ik0] * dirf[ith] : 0.f;
-      float vsnl = t.vsnl[is], vdnl = t.vdnl[is], vsin = t.vsin[is], vdin = t.vdin[is];
-      float vsds = t.vsds[is], vdds = t.vdds[is];
+      float vsnl = cu.vsnl, vdnl = cu.vdnl, vsin = cu.vsin, vdin = cu.vdin;
+      float vsds = cu.vsds, vdds = cu.vdds;
       if (scale) {
         vsnl = isnl * vsnl;
         vdnl = isnl * vdnl;
@@ -1403,12 +1462,12 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
         vsds = isds * vsds;
         vdds = isds * vdds;
       }
-      const float a = t.spec[is];
-      const float vdbt = is < nscut ? cbeta[pp.mapwn[is] - 1] : 0.f;
+      const float a = cu.a;
+      const float vdbt = is < nscut ? cbeta[cu.mw - 1] : 0.f;
       const float vsbt = nscut < 0 ? 0.f : vdbt * a;
       const float vs = (((vsln + vsin) + vsnl) + vsds) + vsbt;
       const float vd = ((vdin + vdnl) + vdds) + vdbt;
-      const float damax = mn(mx(pp.xrel * t.specinit[is], afilt), dam);
+      const float damax = mn(mx(pp.xrel * cu.ai, afilt), dam);
       const float afac = 1.f / mx(__builtin_fabsf(vs / damax), 1.E-10f);
       dtl = mn(dtl, afac / mx(mn(vd, 0.f) * afac + 1.f, 1.E-10f));
     }
@@ -1431,23 +1490,49 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
   // Update (bins below NKH1), W3SDB1 update, flux sums (all bins), in bin order.
   const float vddb = c.v[S_DTDB];
   float phiaw = c.v[S_PHIAW], phibbl = c.v[S_PHIBBL], wc2 = 0.f;
+  struct Upd {
+    float a0, vsin, vdin, vsnl, vdnl, vsds, vdds, vsdb;
+    int mw;
+  };
+  // a lane's next bin is loaded before the current one is computed (the stores below touch
+  // the current bin only)
+  const auto ld = [&](int is) {
+    Upd v{};
+    if (is < nspec) {
+      v.a0 = t.spec[is];
+      v.vsin = t.vsin[is];
+      v.vdin = t.vdin[is];
+      v.mw = is < nscut ? pp.mapwn[is] : 1;
+      if (is < nspech) {
+        v.vsnl = t.vsnl[is];
+        v.vdnl = t.vdnl[is];
+        v.vsds = t.vsds[is];
+        v.vdds = t.vdds[is];
+        v.vsdb = t.vsdb[is];
+      }
+    }
+    return v;
+  };
+  Upd nx = ld(l);
   for (int b0 = 0; b0 < nspec; b0 += G) {
     const int is = b0 + l;
+    const Upd cu = nx;
+    nx = ld(is + G);
     float tpa = 0.f, tpb = 0.f, twc = 0.f, vsin = 0.f;
     if (is < nspec) {
       const int ik0 = is / nth;
       const int ith = is - ik0 * nth;
-      const float a0 = t.spec[is];
-      const float vdbt = is < nscut ? cbeta[pp.mapwn[is] - 1] : 0.f;
+      const float a0 = cu.a0;
+      const float vdbt = is < nscut ? cbeta[cu.mw - 1] : 0.f;
       const float vsbt = nscut < 0 ? 0.f : vdbt * a0;
-      vsin = t.vsin[is];
-      float vdin = t.vdin[is];
+      vsin = cu.vsin;
+      float vdin = cu.vdin;
       float a = a0;
       if (is < nspech) {
         const float w = t.wn1[ik0];
         const float dam = pp.facp / (pp.sig[ik0] * (w * (w * w)));
         float vsln = pp.ln1 ? wnf[ik0] * dirf[ith] : 0.f;
-        float vsnl = t.vsnl[is], vdnl = t.vdnl[is], vsds = t.vsds[is], vdds = t.vdds[is];
+        float vsnl = cu.vsnl, vdnl = cu.vdnl, vsds = cu.vsds, vdds = cu.vdds;
         if (scale) {
           vsnl = isnl * vsnl;
           vdnl = isnl * vdnl;
@@ -1466,7 +1551,7 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
         } else {
           a = mx(a + e1, 0.f);
         }
-        const float e3 = (dt * t.vsdb[is]) / mx(1.f - hdt * vddb, 1.f);
+        const float e3 = (dt * cu.vsdb) / mx(1.f - hdt * vddb, 1.f);
         a = mx(a + e3, 0.f);
         t.spec[is] = a;
       }
@@ -1474,16 +1559,15 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
       const float factor = pp.dden[ik0] / t.cg1[ik0];
       tpa = ((vsin * dt) * factor) / mx(1.f - vdin * dt, 1.f);
       tpb = ((vsbt * dt) * factor) / mx(1.f - vdbt * dt, 1.f);
-      twc = a * factor;
+      // only growing bins add to the whitecap sum; the others add -0 (no change)
+      twc = vsin > 0.f ? a * factor : -0.f;
     }
     const int cnt = nspec - b0;
 #define ST(k) if constexpr ((k) < G)                                   \
   if (k < cnt) {                                \
     phiaw = bc<G, k>(tpa) + phiaw;                 \
     phibbl = phibbl - bc<G, k>(tpb);               \
-    const float vk = bc<G, k>(vsin);               \
-    const float wk = bc<G, k>(twc);                \
-    if (vk > 0.f) wc2 = wc2 + wk;               \
+    wc2 = wc2 + bc<G, k>(twc);                     \
   }
     WW_ROW(ST)
 #undef ST
@@ -1529,12 +1613,12 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
       t.spec[ith + ik * nth] = a;
     }
   wave_sync();
-  sin4<G>(t, q, c);
+  sin4<G, NTH>(t, q, c);
   const bool done = dttot >= 0.9999f * dtg;
   if (q.sintailpar[3] < 0.5f) {
     spr4<G>(t, q, c);
     // the next sub-step's input terms see the u* and z0 of this W3SPR4
-    if (!done) sin4<G>(t, q, c);
+    if (!done) sin4<G, NTH>(t, q, c);
   }
   c.iv[I_DONE] = done ? 1 : 0;
   if (l == 0) {
@@ -1676,13 +1760,16 @@ template <int G>
 inline dim3 groups(int n) { return dim3(static_cast<unsigned>((n + 64 / G - 1) / (64 / G))); }
 
 template <int G>
-inline unsigned lds_of(int nk, int nth) {
-  return static_cast<unsigned>((64 / G) * static_cast<size_t>(Lds(nk, nth).stride) * sizeof(float));
+inline unsigned lds_of(const St4Dev& q) {
+  const Lds o(q.nk, q.nth, q.sintailpar[2] > 0.f);
+  return static_cast<unsigned>((64 / G) * static_cast<size_t>(o.stride) * sizeof(float));
 }
 
 }  // namespace
 
-size_t group_lds_bytes(int nk, int nth) { return lds_of<16>(nk, nth); }
+size_t group_lds_bytes(int nk, int nth) {
+  return 4 * static_cast<size_t>(Lds(nk, nth, true).stride) * sizeof(float);
+}
 
 size_t state_bytes(int nk, int nth, int npts) {
   const size_t n = static_cast<size_t>(npts);
@@ -1691,6 +1778,7 @@ size_t state_bytes(int nk, int nth, int npts) {
   size_t b = 0;
   b += 9 * al(nspec * n * 4);                       // specinit .. vsdb, rnw
   b += al(nspec * n);                               // llws
+  b += al(static_cast<size_t>(nk) * n * 4);         // mft
   b += al(static_cast<size_t>(N_SCAL) * n * 4);
   b += al(static_cast<size_t>(N_ISCAL) * n * 4);
   b += al(n * 4);                                   // kdmean
@@ -1721,6 +1809,7 @@ State carve_state(void* base, int nk, int nth, int npts) {
   s.vsdb = reinterpret_cast<float*>(take(nspec * n * 4));
   s.rnw = reinterpret_cast<float*>(take(nspec * n * 4));
   s.llws = reinterpret_cast<unsigned char*>(take(nspec * n));
+  s.mft = reinterpret_cast<float*>(take(static_cast<size_t>(nk) * n * 4));
   s.sc = reinterpret_cast<float*>(take(static_cast<size_t>(N_SCAL) * n * 4));
   s.isc = reinterpret_cast<int*>(take(static_cast<size_t>(N_ISCAL) * n * 4));
   s.kdmean = reinterpret_cast<float*>(take(n * 4));
@@ -1731,14 +1820,14 @@ State carve_state(void* base, int nk, int nth, int npts) {
 }
 
 hipError_t k_init(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, hipStream_t st) {
-  hipLaunchKernelGGL(init_kernel<16>, groups<16>(s.npts), dim3(64), lds_of<16>(p.nk, p.nth), st, s, q, p, f);
+  hipLaunchKernelGGL(init_kernel<16>, groups<16>(s.npts), dim3(64), lds_of<16>(q), st, s, q, p, f);
   return hipGetLastError();
 }
 
 template <int G>
 hipError_t launch_step(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, int iter, int n,
                        hipStream_t st) {
-  const unsigned lds = lds_of<G>(p.nk, p.nth);
+  const unsigned lds = lds_of<G>(q);
   if (p.nth == 36)
     hipLaunchKernelGGL((step_kernel<G, 36>), groups<G>(n), dim3(64), lds, st, s, q, p, f, iter, n);
   else if (p.nth == 24)
