@@ -181,3 +181,95 @@ def srce_batch(case: dict, tables: dict, dia: Snl1Tables = None, nthreads: int =
     if rc != 0:
         raise RuntimeError("or_srce_batch: unsupported configuration or allocation failure")
     return out
+
+
+# ----------------------------------------------------------------------------------------
+# PR3/UQ propagation: W3XYP3 + W3QCK3, W3KTP3 + W3QCK1/W3QCK2
+# ----------------------------------------------------------------------------------------
+_XYP3_INTS = {"dtg": ctypes.c_float, "flbpi": ctypes.c_int, "nbi": ctypes.c_int, "ncol": ctypes.c_int}
+_KTP3_INTS = {"itstep": ctypes.c_int, "ntimes": ctypes.c_int, "facth": ctypes.c_float,
+              "fack": ctypes.c_float, "nsea": ctypes.c_int, "npts": ctypes.c_int, "ncol": ctypes.c_int}
+
+
+def _declare_prop(L, xyp3_args, ktp3_args):
+    if hasattr(L, "_prop_declared"):
+        return
+    L.or_xyp3_batch.argtypes = [_XYP3_INTS.get(k, ctypes.c_void_p) for k in xyp3_args] + [ctypes.c_int]
+    L.or_xyp3_batch.restype = ctypes.c_int
+    L.or_ktp3_batch.argtypes = [_KTP3_INTS.get(k, ctypes.c_void_p) for k in ktp3_args] + [ctypes.c_int]
+    L.or_ktp3_batch.restype = ctypes.c_int
+    L.or_xyp3_ntloc.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_float] + [ctypes.c_void_p] * 6
+    L.or_xyp3_ntloc.restype = None
+    L.or_prop_const.argtypes = [ctypes.c_int]
+    L.or_prop_const.restype = ctypes.c_float
+    L._prop_declared = True
+
+
+def _argv(case, names, scalars):
+    out = []
+    for k in names:
+        v = case[k]
+        out.append(scalars[k](v) if k in scalars else _vp(v))
+    return out
+
+
+def xyp3_batch(case: dict, nthreads: int = 1):
+    """ww_xyp3_batch on the CPU; returns VA after the call (the case is not modified)."""
+    from ww3gpu import prop_case as pc
+    L = lib()
+    _declare_prop(L, pc.XYP3_ARGS, pc.KTP3_ARGS)
+    c = dict(case)
+    c["va"] = case["va"].copy()
+    L.or_xyp3_batch(*_argv(c, pc.XYP3_ARGS, _XYP3_INTS), nthreads)
+    return c["va"]
+
+
+def xyp3_ntloc(case: dict) -> np.ndarray:
+    from ww3gpu import prop_case as pc
+    L = lib()
+    _declare_prop(L, pc.XYP3_ARGS, pc.KTP3_ARGS)
+    out = np.zeros(case["nspec"], np.int32)
+    L.or_xyp3_ntloc(_vp(case["rpar"]), _vp(case["ipar"]), case["dtg"], _vp(case["sig"]),
+                    _vp(case["ecos"]), _vp(case["esin"]), _vp(case["cx"]), _vp(case["cy"]), _vp(out))
+    return out
+
+
+def prop_const(which: int) -> float:
+    from ww3gpu import prop_case as pc
+    L = lib()
+    _declare_prop(L, pc.XYP3_ARGS, pc.KTP3_ARGS)
+    return L.or_prop_const(which)
+
+
+def ktp3_batch(case: dict, nthreads: int = 1):
+    """ww_ktp3_batch on the CPU; returns (VA, CFLTHMAX, CFLKMAX) after the call."""
+    from ww3gpu import prop_case as pc
+    L = lib()
+    _declare_prop(L, pc.XYP3_ARGS, pc.KTP3_ARGS)
+    c = dict(case)
+    for k in ("va", "cflthmax", "cflkmax"):
+        c[k] = case[k].copy()
+    L.or_ktp3_batch(*_argv(c, pc.KTP3_ARGS, _KTP3_INTS), nthreads)
+    return c["va"], c["cflthmax"], c["cflkmax"]
+
+
+def qck3(mx, my, nx, ny, cfll, trans, q, close, inc, mapact, nact, mapbou, nb0, nb1, nb2):
+    """W3QCK3 on one field; cfll and q are updated in place."""
+    L = lib()
+    L.or_qck3.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 3 + [ctypes.c_int] * 2 + \
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p] + [ctypes.c_int] * 3 + [ctypes.c_void_p]
+    L.or_qck3.restype = None
+    fla = np.zeros(my * (mx + 1), np.float32)
+    L.or_qck3(mx, my, nx, ny, _vp(cfll), _vp(trans), _vp(q), int(close), inc, _vp(mapact), nact,
+              _vp(mapbou), nb0, nb1, nb2, _vp(fla))
+
+
+def qck1(mx, my, nx, ny, cfll, q, close, inc, mapact, nact, mapbou, nb0, nb1, nb2):
+    """W3QCK1 on one field; cfll and q are updated in place."""
+    L = lib()
+    L.or_qck1.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 2 + [ctypes.c_int] * 2 + \
+        [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p] + [ctypes.c_int] * 3 + [ctypes.c_void_p]
+    L.or_qck1.restype = None
+    fla = np.zeros(my * (mx + 1), np.float32)
+    L.or_qck1(mx, my, nx, ny, _vp(cfll), _vp(q), int(close), inc, _vp(mapact), nact,
+              _vp(mapbou), nb0, nb1, nb2, _vp(fla))
