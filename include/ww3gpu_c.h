@@ -319,6 +319,112 @@ long ww3gpu_snl1_lds_bytes(int igrid);
  * Returns OK, NOT_INITIALISED, BAD_SHAPE or KERNEL. */
 int ww3gpu_devlibm_eval(int fn, const float* x, const float* x2, float* y, long n, void* stream);
 
+/* ------------------------------------------------------------------------------------ */
+/* PR3/UQ propagation (kokkos/src/fortran_iface/prop3_c.hpp:17-94).                       */
+/* ------------------------------------------------------------------------------------ */
+
+/* Packet lengths of ww_xyp3_batch (RPAR DTCFL, DTH, XFR, WDCG, WDTH; IPAR NX, NY, NSEA, NK,
+ * NTH, NSPEC, ICLOSE, FLCX, FLCY, FLAGLL, FLCUR; IMAPS NMX0, NMX1, NMX2, NMY0, NMY1, NMY2,
+ * NACT, NCENT; TSCAL DSEC21 of the day, of TBPI0 to TIME, of TBPI0 to TBPIN) and of
+ * ww_ktp3_batch (RPAR DMIN, CTMAX, FACHFA; IPAR NK, NTH, NSPEC, NK2, FLCTH, FLCK, FLCUR;
+ * PSCAL CTHG0S, DEPTH, DDDX, DDDY, CX, CY, DCXDX, DCXDY, DCYDX, DCYDY per point).
+ * Replaces prop3_c.hpp:21 and :75. */
+enum { WW_XYP3_NRPAR = 5, WW_XYP3_NIPAR = 11, WW_XYP3_NIMAPS = 8, WW_XYP3_NTSCAL = 3 };
+enum { WW_KTP3_NRPAR = 3, WW_KTP3_NIPAR = 7, WW_KTP3_NPSCAL = 10 };
+
+/* W3GATH, W3XYP3, W3SCAT for every component ISPEC with on[ISPEC-1] /= 0 in one call: the
+ * ISPEC loop of W3WAVE section 3.6.3 on a regular or curvilinear grid.  Arguments in the
+ * Fortran's layouts (prop3_shim.cpp:936-968 lists them); VA(NSPEC, 0:ncol-1) INOUT.
+ * Returns OK, NOT_INITIALISED, BAD_SHAPE, UNSUPPORTED (tripole grid), TABLE (maps that
+ * W3MAP3 cannot produce: a cell listed twice, a row outside the map's range, MAPCXY not a
+ * permutation), RUNTIME or KERNEL; a refused call writes nothing.
+ * Replaces prop3_c.hpp:25-32. */
+int ww_xyp3_batch(const float* rpar, int nrpar, const int* ipar, int nipar, const int* imaps,
+                  int nimaps, const float* tscal, int ntscal, float dtg, const int* mapsf,
+                  const float* clats, const float* sig, const float* ecos, const float* esin,
+                  const float* dpdx, const float* dpdy, const float* dqdx, const float* dqdy,
+                  const float* gsqrt, const float* atrnx, const float* atrny, const int* mapx2,
+                  const int* mapy2, const int* mapaxy, const int* mapcxy, const int* maptrn,
+                  const int* mapsta, const float* cg, const float* cx, const float* cy,
+                  int flbpi, int nbi, const int* isbpi, const float* bbpi0, const float* bbpin,
+                  const int* on, int ncol, float* va);
+
+/* The resident-VA and MPI forms (prop3_c.hpp:37-68): this library keeps no resident VA and
+ * runs on one GPU; each returns UNSUPPORTED.  A caller with VA in device memory uses
+ * ww3gpu_xyp3_batch_dev / ww3gpu_ktp3_batch_dev. */
+int ww_xyp3_batch_va(int igrid, const float* rpar, int nrpar, const int* ipar, int nipar,
+                     const int* imaps, int nimaps, const float* tscal, int ntscal, float dtg,
+                     const int* mapsf, const float* clats, const float* sig, const float* ecos,
+                     const float* esin, const float* dpdx, const float* dpdy, const float* dqdx,
+                     const float* dqdy, const float* gsqrt, const float* atrnx,
+                     const float* atrny, const int* mapx2, const int* mapy2, const int* mapaxy,
+                     const int* mapcxy, const int* maptrn, const int* mapsta, const float* cg,
+                     const float* cx, const float* cy, int flbpi, int nbi, const int* isbpi,
+                     const float* bbpi0, const float* bbpin, const int* on, float* va, int nspec,
+                     int nsealm);
+int ww_xyp3_batch_mpi(int igrid, const float* rpar, int nrpar, const int* ipar, int nipar,
+                      const int* imaps, int nimaps, const float* tscal, int ntscal, float dtg,
+                      const int* mapsf, const float* clats, const float* sig, const float* ecos,
+                      const float* esin, const float* dpdx, const float* dpdy, const float* dqdx,
+                      const float* dqdy, const float* gsqrt, const float* atrnx,
+                      const float* atrny, const int* mapx2, const int* mapy2, const int* mapaxy,
+                      const int* mapcxy, const int* maptrn, const int* mapsta, const float* cg,
+                      const float* cx, const float* cy, int flbpi, int nbi, const int* isbpi,
+                      const float* bbpi0, const float* bbpin, const int* on, float* va,
+                      int nspec, int nsealm, int naproc, int iaproc, const int* poff,
+                      const float* rbuf, float* sbuf, int nbuf);
+int ww_xyp3_mpi_pack(int igrid, float* va, int nspec, int nsealm, const int* isps, int nisp,
+                     float* buf);
+int ww_xyp3_mpi_unpack(int igrid, float* va, int nspec, int nsealm, const int* isps, int nisp,
+                       const float* buf);
+
+/* W3KTP3 NTIMES times on each of NPTS sea points (W3WAVE sections 3.6.2/3.6.4), VA(NSPEC,
+ * 0:ncol-1), CFLTHMAX/CFLKMAX(ncol-1) INOUT.  Status codes as ww_xyp3_batch.
+ * Replaces prop3_c.hpp:80-85. */
+int ww_ktp3_batch(const float* rpar, int nrpar, const int* ipar, int nipar, const float* sig,
+                  const float* dsip, const float* ecos, const float* esin, const float* ec2,
+                  const float* esc, const float* es2, const int* mapwn, const int* mapth2,
+                  const int* mapwn2, int itstep, int ntimes, float facth, float fack, int nsea,
+                  const float* cg, const float* wn, int npts, const int* jsea, const int* isea,
+                  const float* pscal, int ncol, float* va, float* cflthmax, float* cflkmax);
+/* Replaces prop3_c.hpp:89-94 (see ww_xyp3_batch_va). */
+int ww_ktp3_batch_va(int igrid, const float* rpar, int nrpar, const int* ipar, int nipar,
+                     const float* sig, const float* dsip, const float* ecos, const float* esin,
+                     const float* ec2, const float* esc, const float* es2, const int* mapwn,
+                     const int* mapth2, const int* mapwn2, int itstep, int ntimes, float facth,
+                     float fack, int nsea, const float* cg, const float* wn, int npts,
+                     const int* jsea, const int* isea, const float* pscal, float* va, int nspec,
+                     int nsealm, float* cflthmax, float* cflkmax);
+
+/* Extensions (no counterpart in the reference): ww_xyp3_batch / ww_ktp3_batch with VA (and
+ * the CFL maxima) in device memory, enqueued on `stream` (NULL = the library's stream); the
+ * call returns when the work is done.  The other arrays are host arrays, as in the ABI. */
+int ww3gpu_xyp3_batch_dev(const float* rpar, int nrpar, const int* ipar, int nipar,
+                          const int* imaps, int nimaps, const float* tscal, int ntscal,
+                          float dtg, const int* mapsf, const float* clats, const float* sig,
+                          const float* ecos, const float* esin, const float* dpdx,
+                          const float* dpdy, const float* dqdx, const float* dqdy,
+                          const float* gsqrt, const float* atrnx, const float* atrny,
+                          const int* mapx2, const int* mapy2, const int* mapaxy,
+                          const int* mapcxy, const int* maptrn, const int* mapsta,
+                          const float* cg, const float* cx, const float* cy, int flbpi, int nbi,
+                          const int* isbpi, const float* bbpi0, const float* bbpin,
+                          const int* on, int ncol, float* d_va, void* stream);
+int ww3gpu_ktp3_batch_dev(const float* rpar, int nrpar, const int* ipar, int nipar,
+                          const float* sig, const float* dsip, const float* ecos,
+                          const float* esin, const float* ec2, const float* esc,
+                          const float* es2, const int* mapwn, const int* mapth2,
+                          const int* mapwn2, int itstep, int ntimes, float facth, float fack,
+                          int nsea, const float* cg, const float* wn, int npts, const int* jsea,
+                          const int* isea, const float* pscal, int ncol, float* d_va,
+                          float* d_cflthmax, float* d_cflkmax, void* stream);
+/* The status of the last propagation call. */
+int ww3gpu_prop3_last_error(void);
+/* out[0] GPU ms of the last W3XYP3 call, out[1] its cell-component sub-steps (NX*NY per
+ * component per CFL sub-step), out[2] the most sub-steps of a component, out[3] GPU ms of
+ * the last W3KTP3 call.  n >= 4. */
+int ww3gpu_prop3_stats(double* out, int n);
+
 /* Library version string. */
 const char* ww3gpu_version(void);
 

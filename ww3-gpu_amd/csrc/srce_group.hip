@@ -164,7 +164,7 @@ __device__ inline Grp make_grp(const State& s, const St4Dev& q, int p, float* ld
   const int nk = q.nk, nth = q.nth;
   Grp t{p, static_cast<int>(threadIdx.x) % G, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
         nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-        Lds(nk, nth, q.sintailpar[2] > 0.f)};
+        Lds(nk, nth, q.sintailpar[2] > 0.f), nullptr};
   const size_t nspec = static_cast<size_t>(nk) * nth;
   t.X = lds + static_cast<size_t>(g) * t.o.stride;
   t.W = t.X + nspec;
@@ -1295,7 +1295,7 @@ __global__ __launch_bounds__(64) void init_kernel(State s, St4Dev q, SrceDev pp,
   const int g = static_cast<int>(threadIdx.x) / G;
   const int p = static_cast<int>(blockIdx.x) * (64 / G) + g;
   if (p >= s.npts) return;
-  const int nk = pp.nk, nth = pp.nth, nspec = pp.nspec;
+  const int nk = pp.nk, nspec = pp.nspec;
   const Grp t = make_grp<G>(s, q, p, lds, g);
   const int l = t.l;
   Sc c;

@@ -31,6 +31,12 @@ class WW3GpuError(RuntimeError):
         self.code = code
 
 
+# ww_xyp3_batch and ww_ktp3_batch (prop3_c.hpp:25-32, 80-85)
+_XYP3 = ([c_void_p, c_int] * 4 + [c_float] + [c_void_p] * 21 + [c_int, c_int] + [c_void_p] * 4 +
+         [c_int, c_void_p])
+_KTP3 = ([c_void_p, c_int, c_void_p, c_int] + [c_void_p] * 10 + [c_int, c_int, c_float, c_float, c_int] +
+         [c_void_p] * 2 + [c_int] + [c_void_p] * 3 + [c_int] + [c_void_p] * 3)
+
 # name -> (restype, argtypes).  Pointer arguments are declared as c_void_p so callers
 # can pass either host pointers (numpy .ctypes.data) or device pointers (tensor.data_ptr()).
 SIGNATURES = {
@@ -67,6 +73,18 @@ SIGNATURES = {
     "ww3gpu_srce_timing": (c_int, [c_int, c_int]),
     "ww3gpu_srce_timers": (c_int, [c_int, c_void_p, c_void_p, c_int]),
     "ww3gpu_insin4_tables": (c_int, [c_void_p, c_int, c_void_p, c_int] + [c_void_p] * 7),
+    "ww_xyp3_batch": (c_int, _XYP3),
+    "ww3gpu_xyp3_batch_dev": (c_int, _XYP3 + [c_void_p]),
+    "ww_xyp3_batch_va": (c_int, [c_int] + _XYP3[:-2] + [c_void_p, c_int, c_int]),
+    "ww_xyp3_batch_mpi": (c_int, [c_int] + _XYP3[:-2] + [c_void_p, c_int, c_int, c_int, c_int] +
+                          [c_void_p] * 3 + [c_int]),
+    "ww_xyp3_mpi_pack": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p]),
+    "ww_xyp3_mpi_unpack": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p]),
+    "ww_ktp3_batch": (c_int, _KTP3),
+    "ww3gpu_ktp3_batch_dev": (c_int, _KTP3 + [c_void_p]),
+    "ww_ktp3_batch_va": (c_int, [c_int] + _KTP3[:-4] + [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "ww3gpu_prop3_last_error": (c_int, []),
+    "ww3gpu_prop3_stats": (c_int, [c_void_p, c_int]),
 }
 
 _lib = None
