@@ -1,6 +1,6 @@
 """Benchmark of the spectral-step hot path on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload srce|dia]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload srce|dia|prop]
 
 A step is one pass of the hot path over one batch of synthetic sea points, with every
 input already resident in HBM.
@@ -282,12 +282,115 @@ def bench_srce(args):
     }
 
 
+# W3XYP3: bytes a cell-component sub-step must move at least (read and write VQ, the two CFL
+# numbers of its faces); W3KTP3: per point-call, VA in and out, CG and WN, the 10 scalars.
+PROP_CELL_BYTES = 16
+
+
+def bench_prop(args):
+    import numpy as np
+    import torch
+
+    import oracle as O
+    from ww3gpu import prop, prop_case as pc
+    from ww3gpu.snl1 import init_runtime
+
+    init_runtime()
+    nx, ny, nk, nth = 480, 360, 32, 24
+    c = pc.make_xyp3(nx=nx, ny=ny, nk=nk, nth=nth, seed=2024, land=0.30, dtg=900.0, cfl=0.8,
+                     obstructions=0.02)
+    nsea, nspec = c["nsea"], c["nspec"]
+    ntloc = O.xyp3_ntloc(c)
+    d_va = torch.from_numpy(c["va"].copy()).cuda()
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        prop.xyp3_batch_dev(c, d_va, stream)
+    gpu_ms, wall = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        prop.xyp3_batch_dev(c, d_va, stream)
+        wall.append(time.perf_counter() - t0)
+        gpu_ms.append(prop.stats()["xyp3_ms"])
+    st = prop.stats()
+    cells = st["xyp3_cell_substeps"]
+    ms = float(np.median(gpu_ms))
+    value = cells / (ms * 1e-3)
+    achieved = PROP_CELL_BYTES * cells / (ms * 1e-3) / 1e9
+    # W3KTP3 over the same sea points, one call per part
+    k = pc.make_ktp3(nsea=nsea, nk=nk, nth=nth, seed=2025, ntimes=1)
+    kva = torch.from_numpy(k["va"].copy()).cuda()
+    kth = torch.zeros(nsea, dtype=torch.float32, device="cuda")
+    kk = torch.zeros(nsea, dtype=torch.float32, device="cuda")
+    for _ in range(args.warmup):
+        prop.ktp3_batch_dev(k, kva, kth, kk, stream)
+    kms = []
+    for _ in range(args.steps):
+        prop.ktp3_batch_dev(k, kva, kth, kk, stream)
+        kms.append(prop.stats()["ktp3_ms"])
+    kt_ms = float(np.median(kms))
+    kt_bytes = 4 * (2 * nspec + 2 * (nk + 2) + 10)
+    # CPU baseline: the oracle's W3XYP3 on a bounded sample of the components
+    cores = cpu_cores()
+    sub = dict(c)
+    on = np.zeros(nspec, np.int32)
+    pick = np.arange(0, nspec, max(1, nspec // cores))[:cores]
+    on[pick] = 1
+    sub["on"] = on
+    reps, c0 = 0, time.perf_counter()
+    while True:
+        O.xyp3_batch(sub, nthreads=cores)
+        reps += 1
+        if time.perf_counter() - c0 > args.cpu_seconds:
+            break
+    cpu_s = time.perf_counter() - c0
+    cpu_cells = int(sum(ntloc[i] for i in pick)) * nx * ny * reps
+    return {
+        "metric": "cell-component sub-steps/s for the W3XYP3 propagation step",
+        "value": value,
+        "unit": "cell-component-sub-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (smoothed random land, log-uniform depths, gamma-distributed spectra)",
+        "config": {
+            "workload": f"W3XYP3 + W3QCK3 over all {nspec} components (W3GATH/W3SCAT included), "
+                        f"{nx}x{ny} lon-lat grid at 0.25 deg, NK={nk}, NTH={nth}, DTG=900 s, "
+                        f"CFL 0.8, VA resident in HBM",
+            "nsea": nsea, "nx": nx, "ny": ny, "nk": nk, "nth": nth,
+            "cfl_substeps_max": st["xyp3_substeps"], "cell_component_substeps": cells,
+            "call_wall_ms_median": 1e3 * float(np.median(wall)),
+            "ktp3": {"ms_per_call": kt_ms, "point_calls_per_s": nsea / (kt_ms * 1e-3),
+                     "achieved_gbs": kt_bytes * nsea / (kt_ms * 1e-3) / 1e9,
+                     "workload": f"W3KTP3 + W3QCK1/W3QCK2 on {nsea} points, refraction, "
+                                 f"wavenumber shift and currents on, NTIMES=1"},
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "W3XYP3 call (gather, records, 5 passes per sub-step, store)",
+            "bytes_per_cell_component_substep": PROP_CELL_BYTES,
+        },
+        "cpu_baseline": {
+            "value": cpu_cells / cpu_s, "unit": "cell-component-sub-steps/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{len(pick)} of the {nspec} components x {reps} calls, {cpu_s:.1f} s, "
+                      f"oracle W3XYP3 (C, -O3, OpenMP over components)",
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="srce", choices=["srce", "dia"])
+    ap.add_argument("--workload", default="srce", choices=["srce", "dia", "prop"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -299,6 +402,10 @@ def main():
         args.steps = 500 if args.steps is None else args.steps
         args.warmup = 50 if args.warmup is None else args.warmup
         res = bench_dia(args)
+    elif args.workload == "prop":
+        args.steps = 10 if args.steps is None else args.steps
+        args.warmup = 2 if args.warmup is None else args.warmup
+        res = bench_prop(args)
     else:
         args.steps = 20 if args.steps is None else args.steps
         args.warmup = 3 if args.warmup is None else args.warmup
