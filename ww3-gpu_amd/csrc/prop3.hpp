@@ -35,12 +35,17 @@ struct Comp {
 };
 
 /// A sea point's averaging stencil for one component (prop_xyp3.hpp:983-1059 and the RD
-/// factors of :1160-1163): RD1, MIN(RDI2*CG, 1), RD3, MIN(CG*RDI4, 1) and the offsets packed
-/// as bits: 0 IXC<0 (sum corner), 1 IYC<0, 2 IIXY2 is IXC, 3-5 the same for the difference
-/// corner.
+/// factors of :1160-1163): RD1, MIN(RDI2*CG, 1), RD3, MIN(CG*RDI4, 1).
 struct Rec {
   float rd1, rd2, rd3, rd4;
 };
+
+/// Where a sea point's nine averaging stores go, in the order its loop makes them (centre
+/// first; the central loop's +1..+4, -1..-4 or the near-coast loop's +1, -1, .. -4, each
+/// already sent back to the centre where the reference redirects it): 4 bits per store, the
+/// index of the target's offset in {0, 1, -1, NY, -NY, NY+1, NY-1, -NY+1, -NY-1}; bit 36
+/// marks a near-coast point.
+constexpr int kNearCoastBit = 36;
 
 /// The grid of a W3XYP3 call, device pointers.  Cell IXY = IY + (IX-1)*NY, 1-based.
 struct XyGrid {
@@ -69,9 +74,13 @@ struct XyState {
   int nlist;
   const Comp* comps;  ///< [nlist]
   Rec* rec;           ///< [nlist][NSEA+1]
-  int* code;          ///< [nlist][NSEA+1]
+  uint64_t* tw;       ///< [nlist][NSEA+1]: the store targets (kNearCoastBit)
   float* q[2];        ///< [nlist][NQ] each: FIELD, ping-pong across sub-steps
   float* fla;         ///< [nlist][NF]: FLA(1-NY : NY*NX)
+  /// [nlist][NF] each: VLCFLX(1 : NY*(NX+1)) with the extension column as W3QCK3's closure
+  /// leaves it (column 1 on a closed grid, else 0), and VLCFLY (extension column 0)
+  float* vlcx;
+  float* vlcy;
 };
 
 hipError_t xyp3_gather(const XyGrid& g, const XyState& s, const float* va, hipStream_t st);

@@ -116,7 +116,7 @@ struct XyCtx {
   Derived der;
   std::string derived_key;  ///< the map inputs the tables were built from
   Slot comps;
-  DevBuf rec, code, q0, q1, fla, va;
+  DevBuf rec, tw, q0, q1, fla, vlcx, vlcy, va;
   hipEvent_t ev[2] = {nullptr, nullptr};
   double ms_gpu = 0;
   long long cell_substeps = 0;
@@ -159,6 +159,8 @@ int check_xyp3(const Xy3Args& a, const char* who) {
   if (ip[6] == 8) return fail(WW_KOKKOS_ERR_UNSUPPORTED, std::string(who) + ": tripole grid (W3XYP3 stops there)");
   if (ip[6] != -1 && nx < 2)
     return fail(WW_KOKKOS_ERR_UNSUPPORTED, std::string(who) + ": a closed grid needs NX >= 2");
+  if (ny < 3)  // W3MAP3 lists no active point on rows 1 and NY; the stencil needs NY >= 3
+    return fail(WW_KOKKOS_ERR_UNSUPPORTED, std::string(who) + ": NY < 3");
   const int nxy = nx * ny;
   const int* m = a.imaps;
   if (!(0 <= m[0] && m[0] <= m[1] && m[1] <= m[2] && m[2] <= nxy && 0 <= m[3] && m[3] <= m[4] &&
@@ -388,20 +390,25 @@ int run_xyp3(const Xy3Args& a, float* d_va, hipStream_t st, const char* who) {
   if (e != hipSuccess) return fail(WW_KOKKOS_ERR_RUNTIME, std::string(who) + ": host-to-device copy");
   const size_t nl = static_cast<size_t>(nlist);
   if (x.rec.ensure(nl * (nsea + 1) * sizeof(prop::Rec)) != hipSuccess ||
-      x.code.ensure(nl * (nsea + 1) * iz) != hipSuccess || x.q0.ensure(nl * g.nq * fz) != hipSuccess ||
-      x.q1.ensure(nl * g.nq * fz) != hipSuccess || x.fla.ensure(nl * g.nf * fz) != hipSuccess)
+      x.tw.ensure(nl * (nsea + 1) * sizeof(uint64_t)) != hipSuccess || x.q0.ensure(nl * g.nq * fz) != hipSuccess ||
+      x.q1.ensure(nl * g.nq * fz) != hipSuccess || x.fla.ensure(nl * g.nf * fz) != hipSuccess ||
+      x.vlcx.ensure(nl * g.nf * fz) != hipSuccess || x.vlcy.ensure(nl * g.nf * fz) != hipSuccess)
     return fail(WW_KOKKOS_ERR_RUNTIME, std::string(who) + ": device allocation");
   s.rec = x.rec.as<prop::Rec>();
-  s.code = x.code.as<int>();
+  s.tw = x.tw.as<uint64_t>();
   s.q[0] = x.q0.as<float>();
   s.q[1] = x.q1.as<float>();
   s.fla = x.fla.as<float>();
+  s.vlcx = x.vlcx.as<float>();
+  s.vlcy = x.vlcy.as<float>();
   if (x.ev[0] == nullptr) {
     (void)hipEventCreate(&x.ev[0]);
     (void)hipEventCreate(&x.ev[1]);
   }
   (void)hipEventRecord(x.ev[0], st);
-  e = prop::xyp3_gather(g, s, d_va, st);
+  e = hipMemsetAsync(s.vlcx, 0, nl * g.nf * fz, st);
+  if (e == hipSuccess) e = hipMemsetAsync(s.vlcy, 0, nl * g.nf * fz, st);
+  if (e == hipSuccess) e = prop::xyp3_gather(g, s, d_va, st);
   if (e == hipSuccess) e = prop::xyp3_records(g, s, st);
   const int dttst = fint(a.tscal[0]);
   bool yfirst = (nintl(static_cast<float>(dttst) / a.dtg) % 2) == 0;

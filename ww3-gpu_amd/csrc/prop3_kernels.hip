@@ -65,9 +65,31 @@ __global__ __launch_bounds__(256) void gather_kernel(XyGrid g, XyState s, const 
   }
 }
 
+/// MAPSTX (prop_xyp3.hpp:949-957): MAPSTA with two ghost columns each side, the far side's
+/// on a closed grid, 0 otherwise.
+__device__ __forceinline__ int mapstx(const XyGrid& g, int j) {
+  if (j >= 1 && j <= g.nxy) return g.mapsta[j - 1];
+  if (!g.global) return 0;
+  return j <= 0 ? g.mapsta[j + g.nxy - 1] : g.mapsta[j - g.nxy - 1];
+}
+
+/// Index of a store's offset from its source in {0, 1, -1, NY, -NY, NY+1, NY-1, -NY+1, -NY-1}
+/// (distinct for NY >= 3).
+__device__ __forceinline__ int dir_index(int off, int ny) {
+  if (off == 0) return 0;
+  if (off == 1) return 1;
+  if (off == -1) return 2;
+  if (off == ny) return 3;
+  if (off == -ny) return 4;
+  if (off == ny + 1) return 5;
+  if (off == ny - 1) return 6;
+  if (off == -ny + 1) return 7;
+  return 8;
+}
+
 // ------------------------------------------------------------------------------------
 // The averaging stencil of each sea point for each component (prop_xyp3.hpp:983-1059) with
-// the CG factors of :1160-1163.
+// the CG factors of :1160-1163, and where each of its nine stores lands.
 // ------------------------------------------------------------------------------------
 __device__ __forceinline__ int corner_bits(float dx, float dy, float& rda, float& rdb) {
   int b = (dx < 0.f ? 1 : 0) | (dy < 0.f ? 2 : 0);
@@ -104,24 +126,67 @@ __global__ __launch_bounds__(256) void records_kernel(XyGrid g, XyState s) {
   const float cgi = g.cg[isea * g.cgld + cp.ik];
   const size_t o = static_cast<size_t>(c) * (g.nsea + 1) + isea;
   s.rec[o] = Rec{rdi1, mn(rdi2 * cgi, 1.f), rdi3, mn(cgi * rdi4, 1.f)};
-  s.code[o] = ba | (bb << 3);
+  const int ny = g.ny;
+  const int ixa = (ba & 1) ? -ny : ny, iya = (ba & 2) ? -1 : 1;
+  const int ixb = (bb & 1) ? -ny : ny, iyb = (bb & 2) ? -1 : 1;
+  const int i1 = ixa + iya, i2 = (ba & 4) ? ixa : iya;
+  const int i3 = ixb + iyb, i4 = (bb & 4) ? ixb : iyb;
+  int off[9];
+  off[0] = 0;
+  const bool coast = g.pos[isea] > g.ncent;
+  if (!coast) {  // :1166-1180
+    off[1] = i1;
+    off[2] = i2;
+    off[3] = i3;
+    off[4] = i4;
+    off[5] = -i1;
+    off[6] = -i2;
+    off[7] = -i3;
+    off[8] = -i4;
+  } else {  // :1202-1272
+    const int src = g.mapsf[2 * g.nsea + isea - 1];
+    const auto diag = [&](int i) {
+      const int ixc = isign(ny, i), iyc = i - ixc;
+      return (mapstx(g, src + i) >= 1 && !(mapstx(g, src + ixc) <= 0 && mapstx(g, src + iyc) <= 0)) ? i : 0;
+    };
+    const auto axial = [&](int i) { return mapstx(g, src + i) >= 1 ? i : 0; };
+    off[1] = diag(i1);
+    off[2] = diag(-i1);
+    off[3] = axial(i2);
+    off[4] = axial(-i2);
+    off[5] = diag(i3);
+    off[6] = diag(-i3);
+    off[7] = axial(i4);
+    off[8] = axial(-i4);
+  }
+  uint64_t w = coast ? (1ULL << kNearCoastBit) : 0ULL;
+  for (int j = 0; j < 9; ++j) w |= static_cast<uint64_t>(dir_index(off[j], ny)) << (4 * j);
+  s.tw[o] = w;
+  // VLCFLX, VLCFLY at this cell (prop_xyp3.hpp:1080-1137; the planes are zero elsewhere)
+  const int ixy = g.mapsf[2 * g.nsea + isea - 1];
+  float cxt = (cp.ecosi * cgi) / clat;
+  float cyt = cp.esini * cgi;
+  if (g.flcur) {
+    cxt = cxt + (g.cx[isea] / g.clats[isea]);
+    cyt = cyt + g.cy[isea];
+  }
+  const float cpv = cxt * dpdx + cyt * dpdy;
+  const float cqv = cxt * dqdx + cyt * dqdy;
+  const size_t po = static_cast<size_t>(c) * g.nf;
+  s.vlcx[po + ixy - 1] = cp.dtrad * cpv;
+  s.vlcy[po + ixy - 1] = cp.dtrad * cqv;
+  if (g.global && ixy <= ny) s.vlcx[po + g.nxy + ixy - 1] = cp.dtrad * cpv;  // :1105 closure copy
 }
 
 // ------------------------------------------------------------------------------------
 // Sub-step: averaging (prop_xyp3.hpp:1147-1291) as an ordered gather, then x GSQRT (:1301).
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ int mapstx(const XyGrid& g, int j) {
-  if (j >= 1 && j <= g.nxy) return g.mapsta[j - 1];
-  if (!g.global) return 0;
-  return j <= 0 ? g.mapsta[j + g.nxy - 1] : g.mapsta[j - g.nxy - 1];
-}
 
 /// The value the reference's two scatter loops leave in VQ(t), starting from the 0. of
-/// :1150: the contributions of the neighbours listed for t, each source's stores in its
-/// loop's order.
+/// :1150: the stores of the neighbours listed for t that land on t, each source's stores in
+/// its loop's order (prop_xyp3.hpp:1164-1180 central, :1199-1272 near-coast).
 __device__ float avg_fold(const XyGrid& g, const XyState& s, int c, const float* aq, int t) {
   const int ny = g.ny;
-  const int dir[9] = {0, 1, -1, ny, -ny, ny + 1, ny - 1, -ny + 1, -ny - 1};
   uint64_t ord = g.order[t + ny - 1];
   float v = 0.f;
   const size_t rb = static_cast<size_t>(c) * (g.nsea + 1);
@@ -129,59 +194,33 @@ __device__ float avg_fold(const XyGrid& g, const XyState& s, int c, const float*
     const int d = static_cast<int>(ord & 0xFu);
     if (d == 0xF) break;
     ord >>= 4;
-    const int off = dir[d];
+    const int off = d == 0 ? 0 : (d == 1 ? 1 : d == 2 ? -1 : d == 3 ? ny : d == 4 ? -ny : d == 5 ? ny + 1 : d == 6 ? ny - 1 : d == 7 ? -ny + 1 : -ny - 1);
     const int src = t - off;
-    const int isea = g.cellsea[src];
     const float a = aq[src + ny - 1];
     if (g.maptrn[src - 1] != 0) {  // :1157-1158, :1192-1193 (listed at its own cell only)
       v = a;
       continue;
     }
+    const int isea = g.cellsea[src];
+    const uint64_t w = s.tw[rb + isea];
+    const bool coast = (w >> kNearCoastBit) & 1ULL;
     const Rec r = s.rec[rb + isea];
-    const int code = s.code[rb + isea];
-    const int ixa = (code & 1) ? -ny : ny, iya = (code & 2) ? -1 : 1;
-    const int ixb = (code & 8) ? -ny : ny, iyb = (code & 16) ? -1 : 1;
-    const int i1 = ixa + iya, i2 = (code & 4) ? ixa : iya;
-    const int i3 = ixb + iyb, i4 = (code & 32) ? ixb : iyb;
-    const float rd1 = r.rd1, rd2 = r.rd2, rd3 = r.rd3, rd4 = r.rd4;
-    const float centre = (((3.f - rd2) - rd4) * a) / 3.f;
-    const float c1 = ((rd2 * a) * rd1) / 6.f;
-    const float c2 = (((1.f - rd1) * a) * rd2) / 6.f;
-    const float c3 = ((rd4 * a) * rd3) / 6.f;
-    const float c4 = (((1.f - rd3) * a) * rd4) / 6.f;
-    if (off == 0) v = v + centre;
-    if (g.pos[isea] <= g.ncent) {
-      // :1166-1180: +1, +2, +3, +4, -1, -2, -3, -4
-      if (off == i1) v = v + c1;
-      if (off == i2) v = v + c2;
-      if (off == i3) v = v + c3;
-      if (off == i4) v = v + c4;
-      if (off == -i1) v = v + c1;
-      if (off == -i2) v = v + c2;
-      if (off == -i3) v = v + c3;
-      if (off == -i4) v = v + c4;
-    } else {
-      // :1202-1272: +1, -1, +2, -2, +3, -3, +4, -4, a blocked target going to the centre
-      int ixc = isign(ny, i1), iyc = i1 - ixc;
-      int to;
-      to = (mapstx(g, src + i1) >= 1 && !(mapstx(g, src + ixc) <= 0 && mapstx(g, src + iyc) <= 0)) ? i1 : 0;
-      if (off == to) v = v + c1;
-      to = (mapstx(g, src - i1) >= 1 && !(mapstx(g, src - ixc) <= 0 && mapstx(g, src - iyc) <= 0)) ? -i1 : 0;
-      if (off == to) v = v + c1;
-      to = mapstx(g, src + i2) >= 1 ? i2 : 0;
-      if (off == to) v = v + c2;
-      to = mapstx(g, src - i2) >= 1 ? -i2 : 0;
-      if (off == to) v = v + c2;
-      ixc = isign(ny, i3);
-      iyc = i3 - ixc;
-      to = (mapstx(g, src + i3) >= 1 && !(mapstx(g, src + ixc) <= 0 && mapstx(g, src + iyc) <= 0)) ? i3 : 0;
-      if (off == to) v = v + c3;
-      to = (mapstx(g, src - i3) >= 1 && !(mapstx(g, src - ixc) <= 0 && mapstx(g, src - iyc) <= 0)) ? -i3 : 0;
-      if (off == to) v = v + c3;
-      to = mapstx(g, src + i4) >= 1 ? i4 : 0;
-      if (off == to) v = v + c4;
-      to = mapstx(g, src - i4) >= 1 ? -i4 : 0;
-      if (off == to) v = v + c4;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      if (static_cast<int>((w >> (4 * j)) & 0xFu) != d) continue;
+      const int k = j == 0 ? 0 : (coast ? (j + 1) / 2 : (j - 1) % 4 + 1);
+      float term;
+      if (k == 0)
+        term = (((3.f - r.rd2) - r.rd4) * a) / 3.f;
+      else if (k == 1)
+        term = ((r.rd2 * a) * r.rd1) / 6.f;
+      else if (k == 2)
+        term = (((1.f - r.rd1) * a) * r.rd2) / 6.f;
+      else if (k == 3)
+        term = ((r.rd4 * a) * r.rd3) / 6.f;
+      else
+        term = (((1.f - r.rd3) * a) * r.rd4) / 6.f;
+      v = v + term;
     }
   }
   return v;
@@ -218,25 +257,13 @@ struct Sweep {
   const float* trans;
 };
 
-__device__ __forceinline__ float vel(const XyGrid& g, const Comp& cp, const Sweep& w, int j) {
-  const int isea = g.cellsea[j];
-  if (isea == 0) return 0.f;
-  const float cgi = g.cg[isea * g.cgld + cp.ik];
-  const float clat = g.clats[isea];
-  float cxt = (cp.ecosi * cgi) / clat;
-  float cyt = cp.esini * cgi;
-  if (g.flcur) {
-    cxt = cxt + (g.cx[isea] / g.clats[isea]);
-    cyt = cyt + g.cy[isea];
-  }
-  const int gi = j - 1;
-  const float cpq = w.xdir ? cxt * g.dpdx[gi] + cyt * g.dpdy[gi] : cxt * g.dqdx[gi] + cyt * g.dqdy[gi];
-  return cp.dtrad * cpq;
+/// The CFL plane of a sweep for component c.
+__device__ __forceinline__ const float* cfl_plane(const XyState& s, const Sweep& w, int c, int nf) {
+  return (w.xdir ? s.vlcx : s.vlcy) + static_cast<size_t>(c) * nf;
 }
 
-__device__ __forceinline__ float cfll(const XyGrid& g, const Comp& cp, const Sweep& w, int j) {
-  if (j > g.nxy) return w.close ? vel(g, cp, w, j - g.nxy) : 0.f;
-  return vel(g, cp, w, j);
+__device__ __forceinline__ float cfll(const XyGrid& g, const Sweep& w, const float* cf, int j) {
+  return cf[j - 1];
 }
 
 __device__ __forceinline__ float qat(const XyGrid& g, const Sweep& w, const float* q, int j) {
@@ -247,12 +274,12 @@ __device__ __forceinline__ float qat(const XyGrid& g, const Sweep& w, const floa
   return q[j + g.ny - 1];
 }
 
-__device__ float flux(const XyGrid& g, const Comp& cp, const Sweep& w, const float* q, int ixy) {
+__device__ float flux(const XyGrid& g, const float* cf, const Sweep& w, const float* q, int ixy) {
   const int cls = w.cls[ixy];
   if (cls == 0) return 0.f;
   const int inc = w.inc;
   if (cls == 1) {
-    const float cfl = (cfll(g, cp, w, ixy + inc) + cfll(g, cp, w, ixy)) * 0.5f;
+    const float cfl = (cfll(g, w, cf, ixy + inc) + cfll(g, w, cf, ixy)) * 0.5f;
     const float sg = __builtin_copysignf(1.1f, cfl);
     const int ixyc = ixy - inc * fint(mn(sg, 0.f));
     const float c2 = cfl * cfl;
@@ -277,7 +304,7 @@ __device__ float flux(const XyGrid& g, const Comp& cp, const Sweep& w, const flo
     const float qb2 = qixyc * cfac + (1.f - cfac) * qbr;
     return cfl * qb2;
   }
-  const float cfl = cls == 2 ? cfll(g, cp, w, ixy) : cfll(g, cp, w, ixy + inc);
+  const float cfl = cls == 2 ? cfll(g, w, cf, ixy) : cfll(g, w, cf, ixy + inc);
   const int ixyc = ixy - inc * fint(mn(__builtin_copysignf(1.1f, cfl), 0.f));
   return qat(g, w, q, ixyc) * cfl;
 }
@@ -289,12 +316,13 @@ __global__ __launch_bounds__(256) void flux_kernel(XyGrid g, XyState s, int itlo
   const int k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (k >= g.nf) return;
   const float* q = s.q[itloc & 1] + static_cast<size_t>(c) * g.nq;
+  const float* cf = cfl_plane(s, w, c, g.nf);
   const int i = k + 1 - g.ny;  // FLA(i), i = 1-NY .. NY*NX
   float v;
   if (i <= 0)
-    v = w.close ? flux(g, cp, w, q, i + g.nxy) : 0.f;
+    v = w.close ? flux(g, cf, w, q, i + g.nxy) : 0.f;
   else
-    v = flux(g, cp, w, q, i);
+    v = flux(g, cf, w, q, i);
   s.fla[static_cast<size_t>(c) * g.nf + k] = v;
 }
 
