@@ -73,8 +73,8 @@ struct XyGrid {
 struct XyState {
   int nlist;
   const Comp* comps;  ///< [nlist]
-  Rec* rec;           ///< [nlist][NSEA+1]
-  uint64_t* tw;       ///< [nlist][NSEA+1]: the store targets (kNearCoastBit)
+  Rec* rec;           ///< [nlist][NXY+1], by the source's cell
+  uint64_t* tw;       ///< [nlist][NXY+1]: the store targets (kNearCoastBit)
   float* q[2];        ///< [nlist][NQ] each: FIELD, ping-pong across sub-steps
   float* fla;         ///< [nlist][NF]: FLA(1-NY : NY*NX)
   /// [nlist][NF] each: VLCFLX(1 : NY*(NX+1)) with the extension column as W3QCK3's closure

@@ -389,8 +389,8 @@ int run_xyp3(const Xy3Args& a, float* d_va, hipStream_t st, const char* who) {
   s.comps = static_cast<const Comp*>(put(x.comps, comps.data(), comps.size() * sizeof(Comp)));
   if (e != hipSuccess) return fail(WW_KOKKOS_ERR_RUNTIME, std::string(who) + ": host-to-device copy");
   const size_t nl = static_cast<size_t>(nlist);
-  if (x.rec.ensure(nl * (nsea + 1) * sizeof(prop::Rec)) != hipSuccess ||
-      x.tw.ensure(nl * (nsea + 1) * sizeof(uint64_t)) != hipSuccess || x.q0.ensure(nl * g.nq * fz) != hipSuccess ||
+  if (x.rec.ensure(nl * (nxy + 1) * sizeof(prop::Rec)) != hipSuccess ||
+      x.tw.ensure(nl * (nxy + 1) * sizeof(uint64_t)) != hipSuccess || x.q0.ensure(nl * g.nq * fz) != hipSuccess ||
       x.q1.ensure(nl * g.nq * fz) != hipSuccess || x.fla.ensure(nl * g.nf * fz) != hipSuccess ||
       x.vlcx.ensure(nl * g.nf * fz) != hipSuccess || x.vlcy.ensure(nl * g.nf * fz) != hipSuccess)
     return fail(WW_KOKKOS_ERR_RUNTIME, std::string(who) + ": device allocation");
@@ -406,9 +406,7 @@ int run_xyp3(const Xy3Args& a, float* d_va, hipStream_t st, const char* who) {
     (void)hipEventCreate(&x.ev[1]);
   }
   (void)hipEventRecord(x.ev[0], st);
-  e = hipMemsetAsync(s.vlcx, 0, nl * g.nf * fz, st);
-  if (e == hipSuccess) e = hipMemsetAsync(s.vlcy, 0, nl * g.nf * fz, st);
-  if (e == hipSuccess) e = prop::xyp3_gather(g, s, d_va, st);
+  e = prop::xyp3_gather(g, s, d_va, st);
   if (e == hipSuccess) e = prop::xyp3_records(g, s, st);
   const int dttst = fint(a.tscal[0]);
   bool yfirst = (nintl(static_cast<float>(dttst) / a.dtg) % 2) == 0;

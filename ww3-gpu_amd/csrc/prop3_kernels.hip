@@ -105,12 +105,23 @@ __device__ __forceinline__ int corner_bits(float dx, float dy, float& rda, float
 }
 
 __global__ __launch_bounds__(256) void records_kernel(XyGrid g, XyState s) {
-  const int isea = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + 1;
+  const int ixy = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x) + 1;
   const int c = blockIdx.y;
-  if (isea > g.nsea) return;
+  if (ixy > g.nxy) return;
+  const int ny = g.ny;
+  const size_t po = static_cast<size_t>(c) * g.nf;
+  const int isea = g.cellsea[ixy];
+  if (isea == 0) {  // VLCFLX/VLCFLY are 0 off the sea points (:944-945)
+    s.vlcx[po + ixy - 1] = 0.f;
+    s.vlcy[po + ixy - 1] = 0.f;
+    if (ixy <= ny) {
+      s.vlcx[po + g.nxy + ixy - 1] = 0.f;
+      s.vlcy[po + g.nxy + ixy - 1] = 0.f;
+    }
+    return;
+  }
   const Comp cp = s.comps[c];
-  const int ix = g.mapsf[isea - 1], iy = g.mapsf[g.nsea + isea - 1];
-  const int gi = (ix - 1) * g.ny + iy - 1;
+  const int gi = ixy - 1;
   const float clat = g.clats[isea];
   const float dpdx = g.dpdx[gi], dpdy = g.dpdy[gi], dqdx = g.dqdx[gi], dqdy = g.dqdy[gi];
   float tmpx = cp.fthx / clat, tmpy = cp.fthy;
@@ -124,9 +135,8 @@ __global__ __launch_bounds__(256) void records_kernel(XyGrid g, XyState s) {
   const int ba = corner_bits(dxcgn + dxcgs, dycgn + dycgs, rdi1, rdi2);
   const int bb = corner_bits(dxcgn - dxcgs, dycgn - dycgs, rdi3, rdi4);
   const float cgi = g.cg[isea * g.cgld + cp.ik];
-  const size_t o = static_cast<size_t>(c) * (g.nsea + 1) + isea;
+  const size_t o = static_cast<size_t>(c) * (g.nxy + 1) + ixy;
   s.rec[o] = Rec{rdi1, mn(rdi2 * cgi, 1.f), rdi3, mn(cgi * rdi4, 1.f)};
-  const int ny = g.ny;
   const int ixa = (ba & 1) ? -ny : ny, iya = (ba & 2) ? -1 : 1;
   const int ixb = (bb & 1) ? -ny : ny, iyb = (bb & 2) ? -1 : 1;
   const int i1 = ixa + iya, i2 = (ba & 4) ? ixa : iya;
@@ -144,12 +154,11 @@ __global__ __launch_bounds__(256) void records_kernel(XyGrid g, XyState s) {
     off[7] = -i3;
     off[8] = -i4;
   } else {  // :1202-1272
-    const int src = g.mapsf[2 * g.nsea + isea - 1];
     const auto diag = [&](int i) {
       const int ixc = isign(ny, i), iyc = i - ixc;
-      return (mapstx(g, src + i) >= 1 && !(mapstx(g, src + ixc) <= 0 && mapstx(g, src + iyc) <= 0)) ? i : 0;
+      return (mapstx(g, ixy + i) >= 1 && !(mapstx(g, ixy + ixc) <= 0 && mapstx(g, ixy + iyc) <= 0)) ? i : 0;
     };
-    const auto axial = [&](int i) { return mapstx(g, src + i) >= 1 ? i : 0; };
+    const auto axial = [&](int i) { return mapstx(g, ixy + i) >= 1 ? i : 0; };
     off[1] = diag(i1);
     off[2] = diag(-i1);
     off[3] = axial(i2);
@@ -162,20 +171,21 @@ __global__ __launch_bounds__(256) void records_kernel(XyGrid g, XyState s) {
   uint64_t w = coast ? (1ULL << kNearCoastBit) : 0ULL;
   for (int j = 0; j < 9; ++j) w |= static_cast<uint64_t>(dir_index(off[j], ny)) << (4 * j);
   s.tw[o] = w;
-  // VLCFLX, VLCFLY at this cell (prop_xyp3.hpp:1080-1137; the planes are zero elsewhere)
-  const int ixy = g.mapsf[2 * g.nsea + isea - 1];
+  // VLCFLX, VLCFLY at this cell (prop_xyp3.hpp:1080-1137)
   float cxt = (cp.ecosi * cgi) / clat;
   float cyt = cp.esini * cgi;
   if (g.flcur) {
     cxt = cxt + (g.cx[isea] / g.clats[isea]);
     cyt = cyt + g.cy[isea];
   }
-  const float cpv = cxt * dpdx + cyt * dpdy;
-  const float cqv = cxt * dqdx + cyt * dqdy;
-  const size_t po = static_cast<size_t>(c) * g.nf;
-  s.vlcx[po + ixy - 1] = cp.dtrad * cpv;
-  s.vlcy[po + ixy - 1] = cp.dtrad * cqv;
-  if (g.global && ixy <= ny) s.vlcx[po + g.nxy + ixy - 1] = cp.dtrad * cpv;  // :1105 closure copy
+  const float vx = cp.dtrad * (cxt * dpdx + cyt * dpdy);
+  const float vy = cp.dtrad * (cxt * dqdx + cyt * dqdy);
+  s.vlcx[po + ixy - 1] = vx;
+  s.vlcy[po + ixy - 1] = vy;
+  if (ixy <= ny) {  // the extension column: W3QCK3's closure copy of column 1 (:1105), else 0
+    s.vlcx[po + g.nxy + ixy - 1] = g.global ? vx : 0.f;
+    s.vlcy[po + g.nxy + ixy - 1] = 0.f;
+  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -189,7 +199,7 @@ __device__ float avg_fold(const XyGrid& g, const XyState& s, int c, const float*
   const int ny = g.ny;
   uint64_t ord = g.order[t + ny - 1];
   float v = 0.f;
-  const size_t rb = static_cast<size_t>(c) * (g.nsea + 1);
+  const size_t rb = static_cast<size_t>(c) * (g.nxy + 1);
   for (;;) {
     const int d = static_cast<int>(ord & 0xFu);
     if (d == 0xF) break;
@@ -201,10 +211,9 @@ __device__ float avg_fold(const XyGrid& g, const XyState& s, int c, const float*
       v = a;
       continue;
     }
-    const int isea = g.cellsea[src];
-    const uint64_t w = s.tw[rb + isea];
+    const uint64_t w = s.tw[rb + src];
     const bool coast = (w >> kNearCoastBit) & 1ULL;
-    const Rec r = s.rec[rb + isea];
+    const Rec r = s.rec[rb + src];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
       if (static_cast<int>((w >> (4 * j)) & 0xFu) != d) continue;
@@ -726,7 +735,7 @@ hipError_t xyp3_gather(const XyGrid& g, const XyState& s, const float* va, hipSt
 }
 
 hipError_t xyp3_records(const XyGrid& g, const XyState& s, hipStream_t st) {
-  hipLaunchKernelGGL(records_kernel, dim3((g.nsea + 255) / 256, s.nlist), dim3(256), 0, st, g, s);
+  hipLaunchKernelGGL(records_kernel, dim3((g.nxy + 255) / 256, s.nlist), dim3(256), 0, st, g, s);
   return hipGetLastError();
 }
 
