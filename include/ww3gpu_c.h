@@ -319,6 +319,32 @@ long ww3gpu_snl1_lds_bytes(int igrid);
  * Returns OK, NOT_INITIALISED, BAD_SHAPE or KERNEL. */
 int ww3gpu_devlibm_eval(int fn, const float* x, const float* x2, float* y, long n, void* stream);
 
+/* The per-routine ST4 entries (ww_kokkos_c.hpp:177-196): one routine on npts points of a
+ * committed grid, every per-point scalar an array of npts, point-major arrays; every OUT
+ * argument is copied in as well as out.  Void; the status is read by ww_st4_last_error():
+ * OK, NOT_INITIALISED (grid not committed), BAD_SHAPE (npts < 0 or a null array), RUNTIME,
+ * KERNEL.  The same device routines as W3SRCE's.
+ * W3SDS4: a, srhs, ddiag, brlambda (nspec, npts); k, cg (nk, npts); whitecap (4, npts);
+ * ustar, usdir, depth, dair, dlwmean (npts; DLWMEAN is read only under SDSBCHOICE = 2, which
+ * the commit refuses).  Replaces ww_kokkos_c.hpp:177-180. */
+void ww_sds4(int igrid, int npts, const float* a, const float* k, const float* cg,
+             const float* ustar, const float* usdir, const float* depth, const float* dair,
+             const float* dlwmean, int flogrd57, int flogrd58, float* srhs, float* ddiag,
+             float* brlambda, float* whitecap);
+/* W3SIN4 in the argument order of W3SRCE's call: a, k, s, d, llws (nspec, npts); cg (nk,
+ * npts); the scalars (npts).  Replaces ww_kokkos_c.hpp:184-187. */
+void ww_sin4(int igrid, int npts, const float* a, const float* cg, const float* k,
+             const float* u, const float* ustar, const float* drat, const float* as,
+             const float* usdir, const float* z0, const float* cd, float* tauwx, float* tauwy,
+             float* tauwnx, float* tauwny, float* s, float* d, int* llws);
+/* W3SPR4 with CALC_USTAR: a, llws (nspec, npts); cg, wn (nk, npts); ustar, usdir INOUT.
+ * Replaces ww_kokkos_c.hpp:191-196. */
+void ww_spr4(int igrid, int npts, const float* a, const float* cg, const float* wn,
+             const float* u, const float* udir, const float* tauwx, const float* tauwy,
+             const int* llws, float* ustar, float* usdir, float* emean, float* fmean,
+             float* fmean1, float* wnmean, float* amax, float* cd, float* z0, float* charn,
+             float* fmeanws, float* dlwmean);
+
 /* ------------------------------------------------------------------------------------ */
 /* PR3/UQ propagation (kokkos/src/fortran_iface/prop3_c.hpp:17-94).                       */
 /* ------------------------------------------------------------------------------------ */

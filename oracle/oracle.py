@@ -273,3 +273,32 @@ def qck1(mx, my, nx, ny, cfll, q, close, inc, mapact, nact, mapbou, nb0, nb1, nb
     fla = np.zeros(my * (mx + 1), np.float32)
     L.or_qck1(mx, my, nx, ny, _vp(cfll), _vp(q), int(close), inc, _vp(mapact), nact,
               _vp(mapbou), nb0, nb1, nb2, _vp(fla))
+
+
+def _st4_table_args(case: dict, tables: dict):
+    g = case["grid"]
+    return [_vp(case["st4_rpar"]), _vp(case["st4_ipar"]), _vp(g["sig0"]), _vp(g["sig2"]), _vp(g["dsip0"]),
+            _vp(g["dden"]), _vp(g["dden2"]), _vp(g["esin"]), _vp(g["ecos"]), _vp(g["th"]),
+            _vp(tables["satw"]), _vp(tables["sati"]), tables["nsat"], _vp(tables["cumulw"]), None]
+
+
+def st4_routine(which: str, case: dict, tables: dict, npts: int, arrays: list, ints=()):
+    """or_spr4_batch / or_sin4_batch / or_sds4_batch: `arrays` in the C entry's order after
+    npts (numpy, updated in place), `ints` the trailing int arguments (W3SDS4's flags)."""
+    L = lib()
+    fn = getattr(L, f"or_{which}_batch")
+    n_tab = 15
+    argt = [ctypes.c_void_p] * 12 + [ctypes.c_int] + [ctypes.c_void_p] * 2 + [ctypes.c_int]
+    if which == "sds4":
+        argt += [ctypes.c_void_p] * 8 + [ctypes.c_int] * 2 + [ctypes.c_void_p] * 4
+        args = arrays[:8] + list(ints) + arrays[8:]
+    else:
+        argt += [ctypes.c_void_p] * len(arrays)
+        args = arrays
+    fn.argtypes = argt
+    fn.restype = ctypes.c_int
+    vals = [a if isinstance(a, int) else _vp(a) for a in args]
+    rc = fn(*_st4_table_args(case, tables), npts, *vals)
+    assert n_tab == 15
+    if rc != 0:
+        raise RuntimeError(f"or_{which}_batch refused the configuration")

@@ -1263,3 +1263,98 @@ void or_insin4_tables(const float* st4_rpar, const int* st4_ipar, const float* s
     }
   }
 }
+
+/*
+ * The per-routine entries ww_spr4, ww_sin4, ww_sds4 (kokkos/src/fortran_iface/ww_kokkos_c.hpp:
+ * 177-196): each routine on npts points, every per-point scalar an array of npts.  The ST4
+ * grid arrays as or_srce_batch takes them.  Returns -1 for a configuration the library refuses.
+ */
+static int st4_setup(St4* q, const float* st4_rpar, const int* st4_ipar, const float* sig0,
+                     const float* sig2, const float* dsip0, const float* dden,
+                     const float* dden2, const float* esin, const float* ecos, const float* th,
+                     const float* satw, const int* sati, int nsat, const float* cumulw,
+                     const float* fwtable) {
+  st4_unpack(q, st4_rpar, st4_ipar);
+  if (q->sintailpar[0] > 0.5f || nint(q->ssdsc[0]) != 1 || (q->sswellf[1] != 0.f && fwtable == NULL))
+    return -1;
+  q->sig = sig0;
+  q->sig2 = sig2;
+  q->dsip = dsip0;
+  q->dden = dden;
+  q->dden2 = dden2;
+  q->esin = esin;
+  q->ecos = ecos;
+  q->th = th;
+  q->satw = satw;
+  q->sati = sati;
+  q->nsat = nsat;
+  q->cumulw = cumulw;
+  q->fwtable = fwtable;
+  return 0;
+}
+#define OR_ST4_TABLES                                                                          \
+  const float *st4_rpar, const int *st4_ipar, const float *sig0, const float *sig2,            \
+      const float *dsip0, const float *dden, const float *dden2, const float *esin,            \
+      const float *ecos, const float *th, const float *satw, const int *sati, int nsat,        \
+      const float *cumulw, const float *fwtable
+#define OR_ST4_SETUP                                                                           \
+  St4 q;                                                                                       \
+  if (st4_setup(&q, st4_rpar, st4_ipar, sig0, sig2, dsip0, dden, dden2, esin, ecos, th, satw, \
+                sati, nsat, cumulw, fwtable) != 0)                                             \
+    return -1;                                                                                 \
+  const int nk = q.nk, nspec = q.nspec;
+
+int or_spr4_batch(OR_ST4_TABLES, int npts, const float* a, const float* cg, const float* wn,
+                  const float* u, const float* udir, const float* tauwx, const float* tauwy,
+                  const int* llws, float* ustar, float* usdir, float* emean, float* fmean,
+                  float* fmean1, float* wnmean, float* amax, float* cd, float* z0, float* charn,
+                  float* fmeanws, float* dlwmean) {
+  OR_ST4_SETUP
+  float* w = (float*)malloc(sizeof(float) * 4 * (size_t)nk);
+  for (int i = 0; i < npts; ++i) {
+    Spr4Out o;
+    w3spr4(&q, a + (size_t)i * nspec, cg + (size_t)i * nk, wn + (size_t)i * nk, u[i], udir[i],
+           tauwx[i], tauwy[i], llws + (size_t)i * nspec, &ustar[i], &usdir[i], &wnmean[i],
+           &charn[i], &o, w);
+    emean[i] = o.emean;
+    fmean[i] = o.fmean;
+    fmean1[i] = o.fmean1;
+    amax[i] = o.amax;
+    cd[i] = o.cd;
+    z0[i] = o.z0;
+    fmeanws[i] = o.fmeanws;
+    dlwmean[i] = o.dlwmean;
+  }
+  free(w);
+  return 0;
+}
+
+int or_sin4_batch(OR_ST4_TABLES, int npts, const float* a, const float* cg, const float* k,
+                  const float* u, const float* ustar, const float* drat, const float* as,
+                  const float* usdir, const float* z0, const float* cd, float* tauwx,
+                  float* tauwy, float* tauwnx, float* tauwny, float* s, float* d, int* llws) {
+  OR_ST4_SETUP
+  (void)cd;  /* an argument of W3SIN4 that it never reads */
+  float* w = (float*)malloc(sizeof(float) * 3 * (size_t)nspec);
+  for (int i = 0; i < npts; ++i)
+    w3sin4(&q, a + (size_t)i * nspec, cg + (size_t)i * nk, k + (size_t)i * nspec, u[i], ustar[i],
+           drat[i], as[i], usdir[i], z0[i], &tauwx[i], &tauwy[i], &tauwnx[i], &tauwny[i],
+           s + (size_t)i * nspec, d + (size_t)i * nspec, llws + (size_t)i * nspec, w);
+  free(w);
+  return 0;
+}
+
+int or_sds4_batch(OR_ST4_TABLES, int npts, const float* a, const float* k, const float* cg,
+                  const float* ustar, const float* usdir, const float* depth, const float* dair,
+                  const float* dlwmean, int flogrd57, int flogrd58, float* srhs, float* ddiag,
+                  float* brlambda, float* whitecap) {
+  OR_ST4_SETUP
+  (void)dlwmean;  /* read only under SDSBCHOICE = 2, which is refused */
+  float* w = (float*)malloc(sizeof(float) * (9 * (size_t)nk + 2 * (size_t)nspec));
+  for (int i = 0; i < npts; ++i)
+    w3sds4(&q, a + (size_t)i * nspec, k + (size_t)i * nk, cg + (size_t)i * nk, ustar[i], usdir[i],
+           depth[i], dair[i], flogrd57, flogrd58, srhs + (size_t)i * nspec, ddiag + (size_t)i * nspec,
+           brlambda + (size_t)i * nspec, whitecap + (size_t)i * 4, w);
+  free(w);
+  return 0;
+}

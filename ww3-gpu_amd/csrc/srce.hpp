@@ -145,6 +145,35 @@ hipError_t k_step(const State& s, const St4Dev& q, const SrceDev& p, const StepF
 hipError_t k_finish(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f,
                     hipStream_t st);
 
+/// One ww_spr4 / ww_sin4 / ww_sds4 call on npts points (device copies, point-major).
+struct St4Call {
+  int npts, which;  ///< which: 0 W3SPR4, 1 W3SIN4, 2 W3SDS4
+  int flogrd57, flogrd58;
+  const float* a;   ///< [npts][NSPEC]
+  const float* k;   ///< W3SPR4, W3SDS4: [npts][NK]; W3SIN4: [npts][NSPEC]
+  const float* cg;  ///< [npts][NK]
+  /// per-point scalars read, in the entry's argument order: W3SPR4 U, UDIR, TAUWX, TAUWY;
+  /// W3SIN4 U, USTAR, DRAT, AS, USDIR, Z0; W3SDS4 USTAR, USDIR, DEPTH, DAIR
+  const float* in[6];
+  /// per-point scalars written: W3SPR4 USTAR, USDIR, EMEAN, FMEAN, FMEAN1, WNMEAN, AMAX, CD,
+  /// Z0, CHARN, FMEANWS, DLWMEAN (USTAR and CHARN are also read); W3SIN4 TAUWX, TAUWY,
+  /// TAUWNX, TAUWNY
+  float* out[12];
+  float* s;         ///< W3SIN4 S, W3SDS4 SRHS [npts][NSPEC]
+  float* d;         ///< W3SIN4 D, W3SDS4 DDIAG
+  float* brlambda;  ///< W3SDS4 BRLAMBDA
+  float* whitecap;  ///< W3SDS4 [npts][4] INOUT
+  int* llws;        ///< W3SPR4 in, W3SIN4 out, [npts][NSPEC] 0/1
+  // the library's scratch
+  float* pin;             ///< [npts][N_PIN]
+  unsigned char* llws8;   ///< [npts][NSPEC]
+  float* rnw;             ///< [npts][NSPEC]
+  float* mft;             ///< [npts][NK]
+  float* kband;           ///< [npts][NK]
+};
+/// The routine of io.which on every point (16-lane rows, four points per workgroup).
+hipError_t k_routine(const St4Call& io, const St4Dev& q, hipStream_t st);
+
 /// Counter slots of State::count: the most sub-steps one call may take.  W3SRCE's
 /// sub-steps are at least min(DTMIN, rest) long, so a call takes about DTG/DTMIN of them;
 /// a call that would need more is stopped with an error instead of looping on.

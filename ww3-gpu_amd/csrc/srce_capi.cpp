@@ -400,6 +400,83 @@ bool grid_state(int igrid, srce::St4Dev& out) {
 
 int last_status() { return g_last.load(std::memory_order_relaxed); }
 
+/// One host array of a per-routine call: copied to the device before the kernel, and back
+/// after it when `out` (every OUT argument is copied in too: ww_kokkos_c.hpp:149-152).
+struct HostArr {
+  void* h;
+  size_t bytes;
+  bool out;
+  void* d = nullptr;
+};
+
+/// ww_spr4 / ww_sin4 / ww_sds4: stage the arrays, let `bind` point the call at the device
+/// copies, run the routine, copy the outputs back.
+template <class Bind>
+void routine_with(int igrid, int npts, srce::St4Call& io, std::vector<HostArr>& arrs, const Bind& bind,
+                  const char* who) {
+  srce::St4Dev q;
+  if (!grid_state(igrid, q)) {
+    report(g_last, WW_KOKKOS_ERR_NOT_INITIALISED, (std::string(who) + ": grid not committed").c_str());
+    return;
+  }
+  if (npts < 0) {
+    report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, (std::string(who) + ": npts < 0").c_str());
+    return;
+  }
+  for (const HostArr& x : arrs)
+    if (npts > 0 && x.h == nullptr) {
+      report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, (std::string(who) + ": a null array").c_str());
+      return;
+    }
+  if (npts == 0) {
+    report(g_last, WW_KOKKOS_OK, "");
+    return;
+  }
+  const size_t n = static_cast<size_t>(npts), nspec = q.nspec, nk = q.nk;
+  const auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  size_t total = 0;
+  for (const HostArr& x : arrs) total += al(x.bytes);
+  const size_t sc_pin = al(n * srce::N_PIN * 4), sc_l8 = al(n * nspec), sc_rnw = al(n * nspec * 4),
+               sc_k = al(n * nk * 4);
+  total += sc_pin + sc_l8 + sc_rnw + 2 * sc_k;
+  static DevBuf* buf = new DevBuf();
+  hipStream_t st = runtime().stream;
+  if (buf->ensure(total) != hipSuccess) {
+    report(g_last, WW_KOKKOS_ERR_RUNTIME, (std::string(who) + ": device allocation").c_str());
+    return;
+  }
+  char* cur = static_cast<char*>(buf->p);
+  for (HostArr& x : arrs) {
+    x.d = cur;
+    if (hipMemcpyAsync(cur, x.h, x.bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
+      report(g_last, WW_KOKKOS_ERR_RUNTIME, (std::string(who) + ": host-to-device copy").c_str());
+      return;
+    }
+    cur += al(x.bytes);
+  }
+  io.pin = reinterpret_cast<float*>(cur);
+  cur += sc_pin;
+  io.llws8 = reinterpret_cast<unsigned char*>(cur);
+  cur += sc_l8;
+  io.rnw = reinterpret_cast<float*>(cur);
+  cur += sc_rnw;
+  io.mft = reinterpret_cast<float*>(cur);
+  cur += sc_k;
+  io.kband = reinterpret_cast<float*>(cur);
+  io.npts = npts;
+  bind();
+  hipError_t e = srce::k_routine(io, q, st);
+  if (e == hipSuccess)
+    for (const HostArr& x : arrs)
+      if (x.out && e == hipSuccess) e = hipMemcpyAsync(x.h, x.d, x.bytes, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    report(g_last, WW_KOKKOS_ERR_KERNEL, (std::string(who) + ": " + hipGetErrorString(e)).c_str());
+    return;
+  }
+  report(g_last, WW_KOKKOS_OK, "");
+}
+
 }  // namespace st4h
 
 // ======================================================================================
@@ -785,6 +862,90 @@ int ww_st4_enabled(int which) {
 }
 
 int ww_st4_last_error(void) { return ww3gpu::st4h::last_status(); }
+
+void ww_sds4(int igrid, int npts, const float* a, const float* k, const float* cg, const float* ustar,
+             const float* usdir, const float* depth, const float* dair, const float* dlwmean, int flogrd57,
+             int flogrd58, float* srhs, float* ddiag, float* brlambda, float* whitecap) {
+  using ww3gpu::st4h::HostArr;
+  srce::St4Dev q;
+  const size_t n = npts > 0 ? static_cast<size_t>(npts) : 0;
+  const size_t nspec = ww3gpu::st4h::grid_state(igrid, q) ? q.nspec : 0, nk = nspec ? q.nk : 0;
+  const auto in = [&](const float* p, size_t m) { return HostArr{const_cast<float*>(p), m * 4, false}; };
+  const auto io = [&](float* p, size_t m) { return HostArr{p, m * 4, true}; };
+  std::vector<HostArr> v{in(a, n * nspec), in(k, n * nk), in(cg, n * nk), in(ustar, n), in(usdir, n),
+                         in(depth, n), in(dair, n), in(dlwmean, n), io(srhs, n * nspec), io(ddiag, n * nspec),
+                         io(brlambda, n * nspec), io(whitecap, n * 4)};
+  srce::St4Call c{};
+  c.which = 2;
+  c.flogrd57 = flogrd57 != 0;
+  c.flogrd58 = flogrd58 != 0;
+  const auto run = [&]() {
+    c.a = static_cast<const float*>(v[0].d);
+    c.k = static_cast<const float*>(v[1].d);
+    c.cg = static_cast<const float*>(v[2].d);
+    for (int i = 0; i < 4; ++i) c.in[i] = static_cast<const float*>(v[3 + i].d);
+    c.s = static_cast<float*>(v[8].d);
+    c.d = static_cast<float*>(v[9].d);
+    c.brlambda = static_cast<float*>(v[10].d);
+    c.whitecap = static_cast<float*>(v[11].d);
+  };
+  ww3gpu::st4h::routine_with(igrid, npts, c, v, run, "ww_sds4");
+}
+
+void ww_sin4(int igrid, int npts, const float* a, const float* cg, const float* k, const float* u,
+             const float* ustar, const float* drat, const float* as, const float* usdir, const float* z0,
+             const float* cd, float* tauwx, float* tauwy, float* tauwnx, float* tauwny, float* s, float* d,
+             int* llws) {
+  using ww3gpu::st4h::HostArr;
+  srce::St4Dev q;
+  const size_t n = npts > 0 ? static_cast<size_t>(npts) : 0;
+  const size_t nspec = ww3gpu::st4h::grid_state(igrid, q) ? q.nspec : 0, nk = nspec ? q.nk : 0;
+  const auto in = [&](const void* p, size_t m) { return HostArr{const_cast<void*>(p), m * 4, false}; };
+  const auto io = [&](void* p, size_t m) { return HostArr{p, m * 4, true}; };
+  std::vector<HostArr> v{in(a, n * nspec), in(cg, n * nk), in(k, n * nspec), in(u, n), in(ustar, n),
+                         in(drat, n), in(as, n), in(usdir, n), in(z0, n), in(cd, n), io(tauwx, n), io(tauwy, n),
+                         io(tauwnx, n), io(tauwny, n), io(s, n * nspec), io(d, n * nspec), io(llws, n * nspec)};
+  srce::St4Call c{};
+  c.which = 1;
+  const auto run = [&]() {
+    c.a = static_cast<const float*>(v[0].d);
+    c.cg = static_cast<const float*>(v[1].d);
+    c.k = static_cast<const float*>(v[2].d);
+    for (int i = 0; i < 6; ++i) c.in[i] = static_cast<const float*>(v[3 + i].d);
+    for (int i = 0; i < 4; ++i) c.out[i] = static_cast<float*>(v[10 + i].d);
+    c.s = static_cast<float*>(v[14].d);
+    c.d = static_cast<float*>(v[15].d);
+    c.llws = static_cast<int*>(v[16].d);
+  };
+  ww3gpu::st4h::routine_with(igrid, npts, c, v, run, "ww_sin4");
+}
+
+void ww_spr4(int igrid, int npts, const float* a, const float* cg, const float* wn, const float* u,
+             const float* udir, const float* tauwx, const float* tauwy, const int* llws, float* ustar,
+             float* usdir, float* emean, float* fmean, float* fmean1, float* wnmean, float* amax, float* cd,
+             float* z0, float* charn, float* fmeanws, float* dlwmean) {
+  using ww3gpu::st4h::HostArr;
+  srce::St4Dev q;
+  const size_t n = npts > 0 ? static_cast<size_t>(npts) : 0;
+  const size_t nspec = ww3gpu::st4h::grid_state(igrid, q) ? q.nspec : 0, nk = nspec ? q.nk : 0;
+  const auto in = [&](const void* p, size_t m) { return HostArr{const_cast<void*>(p), m * 4, false}; };
+  const auto io = [&](void* p, size_t m) { return HostArr{p, m * 4, true}; };
+  std::vector<HostArr> v{in(a, n * nspec), in(cg, n * nk), in(wn, n * nk), in(u, n), in(udir, n),
+                         in(tauwx, n), in(tauwy, n), in(llws, n * nspec), io(ustar, n), io(usdir, n),
+                         io(emean, n), io(fmean, n), io(fmean1, n), io(wnmean, n), io(amax, n), io(cd, n),
+                         io(z0, n), io(charn, n), io(fmeanws, n), io(dlwmean, n)};
+  srce::St4Call c{};
+  c.which = 0;
+  const auto run = [&]() {
+    c.a = static_cast<const float*>(v[0].d);
+    c.cg = static_cast<const float*>(v[1].d);
+    c.k = static_cast<const float*>(v[2].d);
+    for (int i = 0; i < 4; ++i) c.in[i] = static_cast<const float*>(v[3 + i].d);
+    c.llws = static_cast<int*>(v[7].d);
+    for (int i = 0; i < 12; ++i) c.out[i] = static_cast<float*>(v[8 + i].d);
+  };
+  ww3gpu::st4h::routine_with(igrid, npts, c, v, run, "ww_spr4");
+}
 
 int ww3gpu_insin4_tables(const float* rpar, int nrpar, const int* ipar, int nipar, const float* sig0,
                          const float* dsip0, const float* ecos, const float* th, float* satw, int* sati,

@@ -1631,6 +1631,87 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
   store_sc(s, t, c);
 }
 
+// ------------------------------------------------------------------------------------
+// The per-routine entries (ww_spr4, ww_sin4, ww_sds4): one routine per point, on the same
+// device functions W3SRCE uses, the point's arrays taken from the call.
+// ------------------------------------------------------------------------------------
+template <int NTH>
+__global__ __launch_bounds__(64) void routine_kernel(St4Call io, St4Dev q) {
+  constexpr int G = 16;
+  extern __shared__ float lds[];
+  const int g = static_cast<int>(threadIdx.x) / G, l = static_cast<int>(threadIdx.x) % G;
+  const int p = static_cast<int>(blockIdx.x) * (64 / G) + g;
+  if (p >= io.npts) return;
+  const int nk = q.nk, nth = q.nth, nspec = q.nspec;
+  const size_t ps = static_cast<size_t>(p) * nspec, pk = static_cast<size_t>(p) * nk;
+  const Lds o(nk, nth, q.sintailpar[2] > 0.f);
+  float* X = lds + static_cast<size_t>(g) * o.stride;
+  const bool sin = io.which == 1, sds = io.which == 2;
+  Grp t{p, l, X, X + nspec, const_cast<float*>(io.a + ps),
+        sin ? io.kband + pk : io.k + pk, io.cg + pk, io.pin + static_cast<size_t>(p) * N_PIN, nullptr,
+        sin ? io.s + ps : nullptr, sin ? io.d + ps : nullptr, nullptr, nullptr,
+        sds ? io.s + ps : nullptr, sds ? io.d + ps : nullptr, nullptr, io.rnw + ps, io.llws8 + ps, o,
+        io.mft + pk};
+  for (int b = l; b < nspec; b += G) X[b] = io.a[ps + b];
+  if (io.which == 0)
+    for (int b = l; b < nspec; b += G) t.llws[b] = io.llws[ps + b] != 0 ? 1 : 0;
+  if (sin)  // W3SIN4 reads K at the first bin of each band only (st4_sin4.hpp:212)
+    for (int ik = l; ik < nk; ik += G) io.kband[pk + ik] = io.k[ps + static_cast<size_t>(ik) * nth];
+  if (sds && io.flogrd58)
+    for (int ik = l; ik < nk; ik += G) io.mft[pk + ik] = mft_integral(io.k[pk + ik], q.sig[ik + 1]);
+  Sc c;
+  for (int i = 0; i < N_SCAL; ++i) c.v[i] = 0.f;
+  for (int i = 0; i < N_ISCAL; ++i) c.iv[i] = 0;
+  float* pin = io.pin + static_cast<size_t>(p) * N_PIN;
+  if (io.which == 0) {
+    if (l == 0) {
+      pin[P_U10ABS] = io.in[0][p];
+      pin[P_U10DIR] = io.in[1][p];
+    }
+    c.v[S_TAUWX] = io.in[2][p];
+    c.v[S_TAUWY] = io.in[3][p];
+    c.v[S_USTAR] = io.out[0][p];
+    c.v[S_CHARN] = io.out[9][p];
+  } else if (sin) {
+    if (l == 0) {
+      pin[P_U10ABS] = io.in[0][p];
+      pin[P_AS] = io.in[3][p];
+      pin[P_U10DIR] = io.in[4][p];
+    }
+    c.v[S_USTAR] = io.in[1][p];
+    c.v[S_DRAT] = io.in[2][p];
+    c.v[S_Z0] = io.in[5][p];
+  } else {
+    if (l == 0) pin[P_DAIR] = io.in[3][p];
+    c.v[S_USTAR] = io.in[0][p];
+    c.v[S_USTDIR] = io.in[1][p];
+    c.v[S_DEPTH] = io.in[2][p];
+  }
+  StepFlags f{1, 0, io.flogrd57, io.flogrd58, 0.f};
+  wave_sync();
+  if (io.which == 0) {
+    spr4<G>(t, q, c);
+    if (l == 0) {
+      const int ids[12] = {S_USTAR, S_USTDIR, S_EMEAN, S_FMEAN, S_FMEAN1, S_WNMEAN, S_AMAX, S_CD,
+                           S_Z0, S_CHARN, S_FMEANWS, S_DLWMEAN};
+      for (int i = 0; i < 12; ++i) io.out[i][p] = c.v[ids[i]];
+    }
+  } else if (sin) {
+    sin4<G, NTH>(t, q, c);
+    for (int b = l; b < nspec; b += G) io.llws[ps + b] = t.llws[b];
+    if (l == 0) {
+      io.out[0][p] = c.v[S_TAUWX];
+      io.out[1][p] = c.v[S_TAUWY];
+      io.out[2][p] = c.v[S_TAUWAX];
+      io.out[3][p] = c.v[S_TAUWAY];
+    }
+  } else {
+    sds4<G, NTH>(t, q, f, c, io.whitecap + static_cast<size_t>(p) * 4);
+    wave_sync();
+    for (int b = l; b < nspec; b += G) io.brlambda[ps + b] = X[b];
+  }
+}
+
 template <int G>
 __global__ __launch_bounds__(64) void finish_kernel(State s, St4Dev q, SrceDev pp, StepFlags f) {
   const int g = static_cast<int>(threadIdx.x) / G;
@@ -1842,6 +1923,19 @@ hipError_t k_step(const State& s, const St4Dev& q, const SrceDev& p, const StepF
   // Many points: a 16-lane row per point (four per wavefront) for throughput.  Few points
   // (the late sub-steps): a wavefront per point, which shortens each point's sub-step.
   return n > kWideBelow ? launch_step<16>(s, q, p, f, iter, n, st) : launch_step<64>(s, q, p, f, iter, n, st);
+}
+
+hipError_t k_routine(const St4Call& io, const St4Dev& q, hipStream_t st) {
+  if (io.npts <= 0) return hipSuccess;
+  const dim3 grid((io.npts + 3) / 4);
+  const unsigned lds = lds_of<16>(q);
+  if (q.nth == 36)
+    hipLaunchKernelGGL((routine_kernel<36>), grid, dim3(64), lds, st, io, q);
+  else if (q.nth == 24)
+    hipLaunchKernelGGL((routine_kernel<24>), grid, dim3(64), lds, st, io, q);
+  else
+    hipLaunchKernelGGL((routine_kernel<0>), grid, dim3(64), lds, st, io, q);
+  return hipGetLastError();
 }
 
 hipError_t k_finish(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, hipStream_t st) {

@@ -73,6 +73,9 @@ SIGNATURES = {
     "ww3gpu_srce_timing": (c_int, [c_int, c_int]),
     "ww3gpu_srce_timers": (c_int, [c_int, c_void_p, c_void_p, c_int]),
     "ww3gpu_insin4_tables": (c_int, [c_void_p, c_int, c_void_p, c_int] + [c_void_p] * 7),
+    "ww_sds4": (None, [c_int, c_int] + [c_void_p] * 8 + [c_int, c_int] + [c_void_p] * 4),
+    "ww_sin4": (None, [c_int, c_int] + [c_void_p] * 17),
+    "ww_spr4": (None, [c_int, c_int] + [c_void_p] * 20),
     "ww_xyp3_batch": (c_int, _XYP3),
     "ww3gpu_xyp3_batch_dev": (c_int, _XYP3 + [c_void_p]),
     "ww_xyp3_batch_va": (c_int, [c_int] + _XYP3[:-2] + [c_void_p, c_int, c_int]),
