@@ -346,6 +346,47 @@ void ww_spr4(int igrid, int npts, const float* a, const float* cg, const float* 
              float* fmeanws, float* dlwmean);
 
 /* ------------------------------------------------------------------------------------ */
+/* The resident spectrum (kokkos/src/fortran_iface/resident_c.hpp:38-103).  One context per  */
+/* grid: VA(NSPEC,0:NSEALM) bound on the grid's first call (va = VA(1,0), nspec = SIZE(VA,1), */
+/* nsealm = UBOUND(VA,2)), a device copy and the state HOST / DEVICE / BOTH saying which copy */
+/* holds the newer bits.  Every call returns the status (BAD_SHAPE when va, nspec or nsealm   */
+/* differ from the first call, or a column is outside 0..nsealm).                            */
+/* ------------------------------------------------------------------------------------ */
+/* Before a host read of VA: DEVICE -> one device-to-host copy, state BOTH.  Replaces :45. */
+int ww_va_host_read(int igrid, float* va, int nspec, int nsealm);
+/* Before a host write of VA: ww_va_host_read, then state HOST.  Replaces :48. */
+int ww_va_host_write(int igrid, float* va, int nspec, int nsealm);
+/* With state DEVICE, copy back only the columns cols[0..ncol) (0-based JSEA, duplicates
+ * allowed); the state is not changed.  Replaces :57. */
+int ww_va_host_read_cols(int igrid, float* va, int nspec, int nsealm, int ncol, const int* cols);
+/* After host writes of exactly the columns cols[0..ncol): with state DEVICE or BOTH those host
+ * columns are copied into the device copy; the state is not changed.  Replaces :68. */
+int ww_va_host_wrote_cols(int igrid, float* va, int nspec, int nsealm, int ncol, const int* cols);
+/* Before a host statement whose only writes of VA zero whole columns: no copy, no change of
+ * state.  Replaces :74. */
+int ww_va_host_update(int igrid, float* va, int nspec, int nsealm);
+/* After it: the device copy's columns cols[0..ncol) = 0 when it holds the newer or equal bits
+ * (DEVICE or BOTH); state unchanged.  Replaces :82-83. */
+int ww_va_device_zero_cols(int igrid, float* va, int nspec, int nsealm, int ncol,
+                           const int* cols);
+/* Inside a kernel entry, before its launch reads the device copy: HOST -> one host-to-device
+ * copy, state BOTH; write = 1 then marks DEVICE.  NOT_INITIALISED for an unbound grid.
+ * Replaces :87. */
+int ww_va_device(int igrid, int write);
+/* The traffic counters, to stderr (also at exit under WW_KOKKOS_VA_COUNT=1).  Replaces :90. */
+void ww_va_report(void);
+/* ww_srce_batch on the resident spectrum: the spectra are the columns jsea[0..npts) (W3WAVE's
+ * JSEA, 1..NSEALM, each once) of VA, read and written in the device copy (state DEVICE
+ * afterwards); every other argument as ww_srce_batch's.  Status in ww_srce_last_error().
+ * Replaces :98-101. */
+void ww_srce_batch_va(int igrid, int it, int inflags24, int flogrd57, int flogrd58, float dtg,
+                      float* va, int nspec, int nsealm, int npts, const int* jsea,
+                      const float* wn1, const float* cg1, const float* pin, float* psc,
+                      float* whitecap, float* taubbl, float* tauice, int* shave);
+/* Extension: the state of grid igrid's resident VA, 0 HOST, 1 DEVICE, 2 BOTH, -1 unbound. */
+int ww3gpu_va_state(int igrid);
+
+/* ------------------------------------------------------------------------------------ */
 /* PR3/UQ propagation (kokkos/src/fortran_iface/prop3_c.hpp:17-94).                       */
 /* ------------------------------------------------------------------------------------ */
 
@@ -375,9 +416,10 @@ int ww_xyp3_batch(const float* rpar, int nrpar, const int* ipar, int nipar, cons
                   int flbpi, int nbi, const int* isbpi, const float* bbpi0, const float* bbpin,
                   const int* on, int ncol, float* va);
 
-/* The resident-VA and MPI forms (prop3_c.hpp:37-68): this library keeps no resident VA and
- * runs on one GPU; each returns UNSUPPORTED.  A caller with VA in device memory uses
- * ww3gpu_xyp3_batch_dev / ww3gpu_ktp3_batch_dev. */
+/* ww_xyp3_batch on the resident VA of grid igrid (ww_va_*; ncol = nsealm + 1): the kernels
+ * read and write the device copy, synced first; the state is DEVICE afterwards.  Replaces
+ * prop3_c.hpp:37-44.  The MPI forms (prop3_c.hpp:50-68) return UNSUPPORTED: one GPU, no
+ * exchange. */
 int ww_xyp3_batch_va(int igrid, const float* rpar, int nrpar, const int* ipar, int nipar,
                      const int* imaps, int nimaps, const float* tscal, int ntscal, float dtg,
                      const int* mapsf, const float* clats, const float* sig, const float* ecos,
@@ -413,7 +455,8 @@ int ww_ktp3_batch(const float* rpar, int nrpar, const int* ipar, int nipar, cons
                   const int* mapwn2, int itstep, int ntimes, float facth, float fack, int nsea,
                   const float* cg, const float* wn, int npts, const int* jsea, const int* isea,
                   const float* pscal, int ncol, float* va, float* cflthmax, float* cflkmax);
-/* Replaces prop3_c.hpp:89-94 (see ww_xyp3_batch_va). */
+/* ww_ktp3_batch on the resident VA of grid igrid, as ww_xyp3_batch_va; CFLTHMAX and CFLKMAX
+ * stay host arrays (NSEALM each).  Replaces prop3_c.hpp:89-94. */
 int ww_ktp3_batch_va(int igrid, const float* rpar, int nrpar, const int* ipar, int nipar,
                      const float* sig, const float* dsip, const float* ecos, const float* esin,
                      const float* ec2, const float* esc, const float* es2, const int* mapwn,

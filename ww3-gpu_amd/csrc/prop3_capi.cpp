@@ -27,6 +27,7 @@
 
 #include "../../include/ww3gpu_c.h"
 #include "prop3.hpp"
+#include "resident.hpp"
 #include "runtime.hpp"
 
 namespace ww3gpu {
@@ -608,7 +609,22 @@ int ww_xyp3_batch_va(int igrid, const float* rpar, int nrpar, const int* ipar, i
                      const int* mapaxy, const int* mapcxy, const int* maptrn, const int* mapsta, const float* cg,
                      const float* cx, const float* cy, int flbpi, int nbi, const int* isbpi, const float* bbpi0,
                      const float* bbpin, const int* on, float* va, int nspec, int nsealm) {
-  return fail(WW_KOKKOS_ERR_UNSUPPORTED, "ww_xyp3_batch_va: the resident VA is not kept by this library");
+  const char* who = "ww_xyp3_batch_va";
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!runtime().ready) return fail(WW_KOKKOS_ERR_NOT_INITIALISED, std::string(who) + " before ww_kokkos_init");
+  if (nrpar != WW_XYP3_NRPAR || nipar != WW_XYP3_NIPAR || nimaps != WW_XYP3_NIMAPS || ntscal != WW_XYP3_NTSCAL)
+    return fail(WW_KOKKOS_ERR_BAD_SHAPE, std::string(who) + ": bad packet");
+  const Xy3Args a{rpar, ipar, imaps, tscal, dtg, mapsf, mapx2, mapy2, mapaxy, mapcxy, maptrn, mapsta, isbpi, on,
+                  clats, sig, ecos, esin, dpdx, dpdy, dqdx, dqdy, gsqrt, atrnx, atrny, cg, cx, cy, bbpi0, bbpin,
+                  flbpi, nbi, nsealm + 1};
+  int rc = check_xyp3(a, who);
+  if (rc != WW_KOKKOS_OK) return rc;
+  if (nspec != ipar[5]) return fail(WW_KOKKOS_ERR_BAD_SHAPE, std::string(who) + ": NSPEC differs from the packet");
+  float* d_va = nullptr;
+  if ((rc = resident::device_va(igrid, va, nspec, nsealm, true, &d_va, who)) != WW_KOKKOS_OK) return rc;
+  rc = run_xyp3(a, d_va, runtime().stream, who);
+  if (rc == WW_KOKKOS_OK) report(g_last, WW_KOKKOS_OK, "");
+  return rc;
 }
 
 int ww_xyp3_batch_mpi(int igrid, const float* rpar, int nrpar, const int* ipar, int nipar, const int* imaps,
@@ -701,7 +717,38 @@ int ww_ktp3_batch_va(int igrid, const float* rpar, int nrpar, const int* ipar, i
                      int ntimes, float facth, float fack, int nsea, const float* cg, const float* wn, int npts,
                      const int* jsea, const int* isea, const float* pscal, float* va, int nspec, int nsealm,
                      float* cflthmax, float* cflkmax) {
-  return fail(WW_KOKKOS_ERR_UNSUPPORTED, "ww_ktp3_batch_va: the resident VA is not kept by this library");
+  const char* who = "ww_ktp3_batch_va";
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!runtime().ready) return fail(WW_KOKKOS_ERR_NOT_INITIALISED, std::string(who) + " before ww_kokkos_init");
+  if (nrpar != WW_KTP3_NRPAR || nipar != WW_KTP3_NIPAR) return fail(WW_KOKKOS_ERR_BAD_SHAPE, std::string(who) + ": bad packet");
+  const Kt3Args a{rpar, ipar, sig, dsip, ecos, esin, ec2, esc, es2, cg, wn, pscal, mapwn, mapth2, mapwn2,
+                  jsea, isea, itstep, ntimes, nsea, npts, nsealm + 1, facth, fack};
+  prop::KtGrid g{};
+  int rc = check_ktp3(a, g, who);
+  if (rc != WW_KOKKOS_OK) return rc;
+  if (nspec != g.nspec || cflthmax == nullptr || cflkmax == nullptr)
+    return fail(WW_KOKKOS_ERR_BAD_SHAPE, std::string(who) + ": NSPEC differs from the packet, or a null array");
+  if (ntimes == 0 || npts == 0) {
+    report(g_last, WW_KOKKOS_OK, "");
+    return WW_KOKKOS_OK;
+  }
+  float* d_va = nullptr;
+  if ((rc = resident::device_va(igrid, va, nspec, nsealm, true, &d_va, who)) != WW_KOKKOS_OK) return rc;
+  KtCtx& x = kt();
+  hipStream_t st = runtime().stream;
+  const size_t bc = static_cast<size_t>(nsealm) * sizeof(float);
+  if (x.cth.ensure(bc) != hipSuccess || x.ck.ensure(bc) != hipSuccess ||
+      hipMemcpyAsync(x.cth.p, cflthmax, bc, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(x.ck.p, cflkmax, bc, hipMemcpyHostToDevice, st) != hipSuccess)
+    return fail(WW_KOKKOS_ERR_RUNTIME, std::string(who) + ": host-to-device copy");
+  rc = run_ktp3(a, g, d_va, x.cth.as<float>(), x.ck.as<float>(), st, who);
+  if (rc != WW_KOKKOS_OK) return rc;
+  if (hipMemcpyAsync(cflthmax, x.cth.p, bc, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(cflkmax, x.ck.p, bc, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return fail(WW_KOKKOS_ERR_RUNTIME, std::string(who) + ": device-to-host copy");
+  report(g_last, WW_KOKKOS_OK, "");
+  return WW_KOKKOS_OK;
 }
 
 int ww3gpu_prop3_last_error(void) { return g_last.load(std::memory_order_relaxed); }

@@ -27,6 +27,7 @@
 #include "runtime.hpp"
 #include "snl1.hpp"
 #include "srce.hpp"
+#include "resident.hpp"
 
 namespace ww3gpu {
 namespace st4h {
@@ -501,6 +502,7 @@ struct Grid {
   DevBuf tables;  ///< sig, dden, ecos, esin, mapwn
   DevBuf state;   ///< the resident state of the last batch
   DevBuf stage;   ///< device copies of a host-array call's arrays
+  DevBuf cols;    ///< the resident-VA columns of a ww_srce_batch_va call
   int* counter = nullptr;  ///< pinned: the host's copy of one sub-step's active count
   int substeps = -1;
   long point_substeps = -1;  ///< sum over the last call's sub-steps of the points integrated
@@ -683,15 +685,17 @@ int check_step(const St4Dev& q, int it, const char* who) {
   return WW_KOKKOS_OK;
 }
 
+/// ww_srce_batch with host arrays; with d_va, the spectra are instead the columns jsea of the
+/// resident VA on the device (ww_srce_batch_va), gathered before and scattered after.
 int batch_host(int igrid, const StepFlags& f, int npts, float* spec, const float* wn1, const float* cg1,
                const float* pin, float* psc, float* whitecap, float* taubbl, float* tauice, int* shave,
-               const char* who) {
+               const char* who, float* d_va = nullptr, const int* jsea = nullptr) {
   Grid* g = nullptr;
   St4Dev q;
   snl1::GridView dv;
   int rc = resolve(igrid, who, g, q, dv);
   if (rc != WW_KOKKOS_OK) return rc;
-  if (!args_ok(npts, spec, wn1, cg1, pin, psc, whitecap, taubbl, tauice, shave)) {
+  if (!args_ok(npts, d_va != nullptr ? d_va : spec, wn1, cg1, pin, psc, whitecap, taubbl, tauice, shave)) {
     report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, (std::string(who) + ": npts < 0 or a null array").c_str());
     return WW_KOKKOS_ERR_BAD_SHAPE;
   }
@@ -723,7 +727,15 @@ int batch_host(int igrid, const StepFlags& f, int npts, float* spec, const float
   const auto h2d = [&](void* d, const void* h, size_t bytes) {
     return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
   };
-  if (!h2d(d_spec, spec, nspec * n * 4) || !h2d(d_wn1, wn1, nk * n * 4) || !h2d(d_cg1, cg1, nk * n * 4) ||
+  if (d_va != nullptr) {
+    if (g->cols.ensure(n * sizeof(int)) != hipSuccess || !h2d(g->cols.p, jsea, n * sizeof(int)) ||
+        resident::gather_cols(d_va, static_cast<int>(nspec), g->cols.as<int>(), npts, d_spec, st) != hipSuccess) {
+      report(g_last, WW_KOKKOS_ERR_RUNTIME, (std::string(who) + ": gathering the resident columns").c_str());
+      return WW_KOKKOS_ERR_RUNTIME;
+    }
+  }
+  if ((d_va == nullptr && !h2d(d_spec, spec, nspec * n * 4)) || !h2d(d_wn1, wn1, nk * n * 4) ||
+      !h2d(d_cg1, cg1, nk * n * 4) ||
       !h2d(d_pin, pin, WW_SRCE_NPIN * n * 4) || !h2d(d_psc, psc, WW_SRCE_NPSC * n * 4) ||
       !h2d(d_wc, whitecap, 4 * n * 4) || !h2d(d_tb, taubbl, 2 * n * 4) || !h2d(d_ti, tauice, 2 * n * 4)) {
     report(g_last, WW_KOKKOS_ERR_RUNTIME, (std::string(who) + ": host-to-device copy").c_str());
@@ -737,7 +749,11 @@ int batch_host(int igrid, const StepFlags& f, int npts, float* spec, const float
   const auto d2h = [&](void* h, const void* d, size_t bytes) {
     return hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
   };
-  if (!d2h(spec, d_spec, nspec * n * 4) || !d2h(psc, d_psc, WW_SRCE_NPSC * n * 4) ||
+  const bool spec_back =
+      d_va != nullptr
+          ? resident::scatter_cols(d_va, static_cast<int>(nspec), g->cols.as<int>(), npts, d_spec, st) == hipSuccess
+          : d2h(spec, d_spec, nspec * n * 4);
+  if (!spec_back || !d2h(psc, d_psc, WW_SRCE_NPSC * n * 4) ||
       !d2h(whitecap, d_wc, 4 * n * 4) || !d2h(taubbl, d_tb, 2 * n * 4) || !d2h(tauice, d_ti, 2 * n * 4) ||
       !d2h(shave, d_sh, n * 4) || hipStreamSynchronize(st) != hipSuccess) {
     report(g_last, WW_KOKKOS_ERR_RUNTIME, (std::string(who) + ": device-to-host copy").c_str());
@@ -1079,6 +1095,37 @@ void ww_srce_batch(int igrid, int it, int inflags24, int flogrd57, int flogrd58,
   const srce::StepFlags f{it, inflags24 != 0, flogrd57 != 0, flogrd58 != 0, dtg};
   (void)ww3gpu::srceh::batch_host(igrid, f, npts, spec, wn1, cg1, pin, psc, whitecap, taubbl, tauice, shave,
                                   "ww_srce_batch");
+}
+
+void ww_srce_batch_va(int igrid, int it, int inflags24, int flogrd57, int flogrd58, float dtg, float* va,
+                      int nspec, int nsealm, int npts, const int* jsea, const float* wn1, const float* cg1,
+                      const float* pin, float* psc, float* whitecap, float* taubbl, float* tauice, int* shave) {
+  using namespace ww3gpu::srceh;
+  const char* who = "ww_srce_batch_va";
+  Grid* g = nullptr;
+  srce::St4Dev q;
+  snl1::GridView dv;
+  if (resolve(igrid, who, g, q, dv) != WW_KOKKOS_OK) return;
+  if (nspec != q.nspec || npts < 0 || npts > nsealm || (npts > 0 && jsea == nullptr)) {
+    report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww_srce_batch_va: NSPEC differs from the grid, or a bad point list");
+    return;
+  }
+  std::vector<char> seen(static_cast<size_t>(nsealm) + 1, 0);
+  for (int i = 0; i < npts; ++i) {
+    if (jsea[i] < 1 || jsea[i] > nsealm || seen[jsea[i]]) {
+      report(g_last, WW_KOKKOS_ERR_BAD_SHAPE, "ww_srce_batch_va: a JSEA outside 1..NSEALM or listed twice");
+      return;
+    }
+    seen[jsea[i]] = 1;
+  }
+  float* d_va = nullptr;
+  const int rc = ww3gpu::resident::device_va(igrid, va, nspec, nsealm, true, &d_va, who);
+  if (rc != WW_KOKKOS_OK) {
+    report(g_last, rc, "ww_srce_batch_va: resident VA");
+    return;
+  }
+  const srce::StepFlags f{it, inflags24 != 0, flogrd57 != 0, flogrd58 != 0, dtg};
+  (void)batch_host(igrid, f, npts, nullptr, wn1, cg1, pin, psc, whitecap, taubbl, tauice, shave, who, d_va, jsea);
 }
 
 void ww_srce(int igrid, int it, int inflags24, int flogrd57, int flogrd58, float* spec, const float* wn1,

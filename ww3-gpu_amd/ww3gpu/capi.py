@@ -76,6 +76,16 @@ SIGNATURES = {
     "ww_sds4": (None, [c_int, c_int] + [c_void_p] * 8 + [c_int, c_int] + [c_void_p] * 4),
     "ww_sin4": (None, [c_int, c_int] + [c_void_p] * 17),
     "ww_spr4": (None, [c_int, c_int] + [c_void_p] * 20),
+    "ww_va_host_read": (c_int, [c_int, c_void_p, c_int, c_int]),
+    "ww_va_host_write": (c_int, [c_int, c_void_p, c_int, c_int]),
+    "ww_va_host_read_cols": (c_int, [c_int, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "ww_va_host_wrote_cols": (c_int, [c_int, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "ww_va_host_update": (c_int, [c_int, c_void_p, c_int, c_int]),
+    "ww_va_device_zero_cols": (c_int, [c_int, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "ww_va_device": (c_int, [c_int, c_int]),
+    "ww_va_report": (None, []),
+    "ww_srce_batch_va": (None, [c_int] * 5 + [c_float, c_void_p, c_int, c_int, c_int] + [c_void_p] * 9),
+    "ww3gpu_va_state": (c_int, [c_int]),
     "ww_xyp3_batch": (c_int, _XYP3),
     "ww3gpu_xyp3_batch_dev": (c_int, _XYP3 + [c_void_p]),
     "ww_xyp3_batch_va": (c_int, [c_int] + _XYP3[:-2] + [c_void_p, c_int, c_int]),
