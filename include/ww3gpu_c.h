@@ -49,9 +49,23 @@ int ww_kokkos_init(int comm_f);
 void ww_kokkos_finalize(void);
 
 /* Replaces ww_kokkos_c.hpp:875 (ww_kokkos_flag; resolution rules
- * kokkos/src/fortran_iface/part_flags.cpp:1-60): 1 if the environment variable
- * `name` was "1" (or WW_KOKKOS_ALL=1 and name is a part flag) when first asked, else 0. */
+ * kokkos/src/fortran_iface/part_flags.cpp:99-112), read once per name: 1 when the variable
+ * `name` is "1" or unset and named by the WW_KOKKOS_PRESET preset, 2 when unset and
+ * WW_KOKKOS_ALL is "1" (except the opt-in parts XYP1, KTP1, XYP2, KTP2, XYPUG, XYP3, KTP3, OUTG,
+ * IOPE, UPDV, CPRT, ST4_TABLES), else 0. */
 int ww_kokkos_flag(const char* name);
+
+/* Replaces ww_kokkos_c.hpp:879: WW_KOKKOS_PRESET 0 unset or empty, 1 a known preset
+ * ("resident": SRCE, XYP3, KTP3, OUTG, IOPE, UPDV, CPRT), -1 any other name. */
+int ww_kokkos_preset_status(void);
+/* Replaces ww_kokkos_c.hpp:883: 1 when WW_KOKKOS_STRICT is "1" (read once). */
+int ww_kokkos_strict(void);
+/* Replaces ww_kokkos_c.hpp:888: the counter of a part's calls left on the Fortran (name at
+ * most 24 characters), created on first use, printed at exit; -1 when the name is too long or
+ * the table (64) is full.  Call serially. */
+int ww_kokkos_fallback_slot(const char* part);
+/* Replaces ww_kokkos_c.hpp:891: one more count for the slot. */
+void ww_kokkos_fallback_add(int slot);
 
 /* ------------------------------------------------------------------------------------ */
 /* W3SNL1, the DIA                                                                       */

@@ -86,6 +86,45 @@ def test_flag_resolution(monkeypatch):
     assert L.ww_kokkos_flag(None) == 0
 
 
+_FLAG_PROBE = """
+import ctypes, sys
+L = ctypes.CDLL(sys.argv[1])
+names = sys.argv[2].split(",")
+vals = [L.ww_kokkos_flag(n.encode()) for n in names]
+slot = L.ww_kokkos_fallback_slot(b"W3SRCE")
+L.ww_kokkos_fallback_add(slot)
+L.ww_kokkos_fallback_add(slot)
+extra = [L.ww_kokkos_fallback_slot(b"W3SRCE"), L.ww_kokkos_fallback_slot(b"X" * 25)]
+print(vals, L.ww_kokkos_preset_status(), L.ww_kokkos_strict(), slot, extra)
+"""
+
+
+def _flags_in(env, names):
+    import subprocess
+    import sys
+    e = {k: v for k, v in os.environ.items() if not k.startswith("WW_KOKKOS")}
+    e.update(env)
+    r = subprocess.run([sys.executable, "-c", _FLAG_PROBE, capi.LIB_PATH, ",".join(names)],
+                       env=e, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    return r.stdout.strip(), r.stderr
+
+
+def test_flag_resolution_rules():
+    """kokkos/src/fortran_iface/part_flags.cpp:97-135: a set flag decides alone; unset, a preset
+    naming the part gives 1, opt-in parts ignore the master, WW_KOKKOS_ALL=1 gives 2."""
+    names = ["WW_KOKKOS_SRCE", "WW_KOKKOS_SNL1", "WW_KOKKOS_XYP3", "WW_KOKKOS_SIN4"]
+    out, err = _flags_in({"WW_KOKKOS_ALL": "1", "WW_KOKKOS_SIN4": "0"}, names)
+    assert out == "[2, 2, 0, 0] 0 0 0 [0, -1]"
+    assert "W3SRCE=2" in err
+    out, _ = _flags_in({"WW_KOKKOS_PRESET": "resident", "WW_KOKKOS_STRICT": "1"}, names)
+    assert out == "[1, 0, 1, 0] 1 1 0 [0, -1]"
+    out, _ = _flags_in({"WW_KOKKOS_PRESET": "resident", "WW_KOKKOS_ALL": "1", "WW_KOKKOS_XYP3": "yes"}, names)
+    assert out == "[1, 2, 0, 2] 1 0 0 [0, -1]"
+    out, _ = _flags_in({"WW_KOKKOS_PRESET": "nosuch"}, names)
+    assert out == "[0, 0, 0, 0] -1 0 0 [0, -1]"
+
+
 def declared_arity():
     src = open(os.path.join(ROOT, "include", "ww3gpu_c.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)

@@ -32,6 +32,12 @@ void report(std::atomic<int>& slot, int code, const char* what);
 /// Register a function that frees a module's grid contexts; ww_kokkos_finalize runs them.
 void on_finalize(std::function<void()> fn);
 
+/// Run-time flag resolution (ww_kokkos_flag and its helpers): see runtime.cpp.
+int resolve_flag(const char* name);
+int preset_status();
+int fallback_slot(const char* part);
+void fallback_add(int slot);
+
 /// A device allocation that only grows.
 struct DevBuf {
   void* p = nullptr;

@@ -215,7 +215,7 @@ void ww_snl1(int igrid, int npts, const float* a, const float* cg, const float* 
   report(g_last, WW_KOKKOS_OK, "");
 }
 
-int ww_snl1_enabled(void) { return ww_kokkos_flag("WW_KOKKOS_SNL1"); }
+int ww_snl1_enabled(void) { return ww_kokkos_flag("WW_KOKKOS_SNL1") != 0 ? 1 : 0; }
 
 int ww_snl1_last_error(void) { return g_last.load(std::memory_order_relaxed); }
 

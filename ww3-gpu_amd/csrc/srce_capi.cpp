@@ -869,10 +869,10 @@ int ww_st4_commit(int igrid) {
 
 int ww_st4_enabled(int which) {
   switch (which) {
-    case 1: return ww_kokkos_flag("WW_KOKKOS_SDS4");
-    case 2: return ww_kokkos_flag("WW_KOKKOS_SIN4");
-    case 3: return ww_kokkos_flag("WW_KOKKOS_SPR4");
-    case 4: return ww_kokkos_flag("WW_KOKKOS_ST4_TABLES");
+    case 1: return ww_kokkos_flag("WW_KOKKOS_SDS4") != 0 ? 1 : 0;
+    case 2: return ww_kokkos_flag("WW_KOKKOS_SIN4") != 0 ? 1 : 0;
+    case 3: return ww_kokkos_flag("WW_KOKKOS_SPR4") != 0 ? 1 : 0;
+    case 4: return ww_kokkos_flag("WW_KOKKOS_ST4_TABLES") != 0 ? 1 : 0;
     default: return 0;
   }
 }
@@ -1155,7 +1155,7 @@ void ww_srce(int igrid, int it, int inflags24, int flogrd57, int flogrd58, float
   *shave = sh;
 }
 
-int ww_srce_enabled(void) { return ww_kokkos_flag("WW_KOKKOS_SRCE"); }
+int ww_srce_enabled(void) { return ww_kokkos_flag("WW_KOKKOS_SRCE") != 0 ? 1 : 0; }
 
 int ww_srce_last_error(void) { return ww3gpu::srceh::g_last.load(std::memory_order_relaxed); }
 

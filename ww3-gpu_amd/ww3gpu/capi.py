@@ -43,6 +43,10 @@ SIGNATURES = {
     "ww_kokkos_init": (c_int, [c_int]),
     "ww_kokkos_finalize": (None, []),
     "ww_kokkos_flag": (c_int, [c_char_p]),
+    "ww_kokkos_preset_status": (c_int, []),
+    "ww_kokkos_strict": (c_int, []),
+    "ww_kokkos_fallback_slot": (c_int, [c_char_p]),
+    "ww_kokkos_fallback_add": (None, [c_int]),
     "ww_snl1_init": (c_int, [c_int, c_int, c_int] + [c_float] * 10 + [c_void_p]),
     "ww_snl1": (None, [c_int, c_int] + [c_void_p] * 5),
     "ww_snl1_enabled": (c_int, []),
