@@ -802,137 +802,132 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
   float* bth0w = t.W + t.o.bth0;
 
   if constexpr (NTH > 0) {
-    // Saturation, MAXVAL (or the isotropic sum), DDIAG, PB -> BRLAMBDA (into X) and SRHS, band
-    // by band, the lanes over directions (ITH = pass*G + l).  MAXVAL is an exact parallel
-    // reduction: the first maximum in ITH order wins a tie, NaN values are skipped, NaN when
-    // there is no other value, which is what the in-order scan of the reference returns.
+    // Saturation, MAXVAL (or the isotropic sum), DDIAG, PB -> BRLAMBDA (into X) and SRHS in
+    // three passes with no reduction across lanes inside a band loop.
+    //  1. B for every bin of the bands from IK1 up: one direction per lane and pass, its
+    //     weights and indices in registers for all bands; B goes to the point's SRHS array
+    //     (the lane's own bins), then into X once every band has been read.
+    //  2. One band per lane: MAXVAL of B in ITH order (a NaN is skipped, the first non-NaN
+    //     value starts the scan, a later value replaces it only when greater: NaN when the
+    //     band holds nothing else), which is what the in-order scan of the reference returns;
+    //     or the isotropic sum of A.  The band's COEF1, SDIAGISO and COEF2 go to LDS.
+    //  3. Every bin: DDIAG, PB, BRLAMBDA, SRHS.
     constexpr int NSAT = 2 * (NTH / 2 - 1) + 1;
     constexpr int NP = (NTH + G - 1) / G;
     const bool fixed = q.nsat == NSAT;
-    // One direction per lane (a wavefront per point): its weights and indices for every band.
-    constexpr bool HOIST = NP == 1;
-    float wh[HOIST ? NSAT : 1];
-    int xh[HOIST ? NSAT : 1];
-    if constexpr (HOIST) {
-      if (fixed && l < NTH) {
+    const int kb = ik1 - 1 < nk ? ik1 - 1 : nk;  // first band of the saturation
+    if (!iso) {
+#pragma unroll 1
+      for (int pp = 0; pp < NP; ++pp) {
+        const int ith = pp * G + l;
+        if (ith >= NTH) break;
+        if (fixed) {
+          float wr[NSAT];
+          int xr[NSAT];
+          const float* w = q.satw + NSAT * ith;
+          const int* x = q.sati + NSAT * ith;
 #pragma unroll
-        for (int j = 0; j < NSAT; ++j) {
-          wh[j] = q.satw[q.nsat * l + j];
-          xh[j] = q.sati[q.nsat * l + j];
+          for (int j = 0; j < NSAT; ++j) {
+            wr[j] = w[j];
+            xr[j] = x[j];
+          }
+          for (int ik0 = kb; ik0 < nk; ++ik0) {
+            const float kk = t.wn1[ik0];
+            const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;
+            const float* xb = t.X + ik0 * NTH - 1;
+            float acc = 0.f;
+#pragma unroll
+            for (int j = 0; j < NSAT; ++j) acc = acc + wr[j] * xb[xr[j]];
+            t.vsds[ik0 * NTH + ith] = acc * facsat;
+          }
+        } else {
+          const float* w = q.satw + q.nsat * ith;
+          const int* x = q.sati + q.nsat * ith;
+          for (int ik0 = kb; ik0 < nk; ++ik0) {
+            const float kk = t.wn1[ik0];
+            const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;
+            const float* xb = t.X + ik0 * NTH - 1;
+            float acc = 0.f;
+            for (int j = 0; j < q.nsat; ++j) acc = acc + w[j] * xb[x[j]];
+            t.vsds[ik0 * NTH + ith] = acc * facsat;
+          }
         }
       }
+      wave_sync();  // every gather of A is done: X takes B
+#pragma unroll 1
+      for (int pp = 0; pp < NP; ++pp) {
+        const int ith = pp * G + l;
+        if (ith >= NTH) break;
+        for (int ik0 = kb; ik0 < nk; ++ik0) t.X[ik0 * NTH + ith] = t.vsds[ik0 * NTH + ith];
+      }
+      wave_sync();
     }
-    for (int ik0 = 0; ik0 < nk; ++ik0) {
-      const int is0 = ik0 * NTH;
-      if (ik0 + 1 < ik1) {
-        for (int ith = l; ith < NTH; ith += G) {
-          const float a = t.X[is0 + ith];
-          t.vdds[is0 + ith] = 0.f;
-          t.vsds[is0 + ith] = 0.f * a;  // SRHS = DDIAG * A with DDIAG = 0
-          t.X[is0 + ith] = 0.f;         // BTH = 0 there, so PB = MAX(-EPSR, 0)**2 * 28.16 = 0
-        }
-        if (l == 0) bth0w[ik0] = 0.f;
+    float* c1w = t.W + t.o.vb;     // COEF1 per band (the isotropic band sums come later)
+    float* sdw = t.W + t.o.rniso;  // SDIAGISO per band
+    float* c2w = t.W + t.o.sat;    // COEF2 per band (the saturated-band list comes later)
+    for (int ik0 = l; ik0 < nk; ik0 += G) {
+      if (ik0 < kb) {
+        bth0w[ik0] = 0.f;
         continue;
       }
-      const float kk = t.wn1[ik0];
-      const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;
-      float bv[NP], av[NP];
+      const float* xb = t.X + ik0 * NTH;
       float bth0;
       if (iso) {
+        const float kk = t.wn1[ik0];
+        const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;
         float asum = 0.f;
-        for (int ith = 0; ith < NTH; ++ith) asum = asum + t.X[is0 + ith];
+        for (int ith = 0; ith < NTH; ++ith) asum = asum + xb[ith];
         bth0 = asum * facsat;
-#pragma unroll
-        for (int pp = 0; pp < NP; ++pp) {
-          const int ith = pp * G + l;
-          av[pp] = ith < NTH ? t.X[is0 + ith] : 0.f;
-          bv[pp] = bth0;
-        }
       } else {
-        const float* xb = t.X + is0 - 1;
-        float best = 0.f;
-        int bi = 0x7fffffff;  // no candidate yet
-#pragma unroll
-        for (int pp = 0; pp < NP; ++pp) {
-          const int ith = pp * G + l;
-          float b = 0.f;
-          av[pp] = 0.f;
-          if (ith < NTH) {
-            av[pp] = t.X[is0 + ith];
-            float acc = 0.f;
-            if (fixed) {
-              float wr[NSAT], ar[NSAT];
-              if constexpr (HOIST) {
-#pragma unroll
-                for (int j = 0; j < NSAT; ++j) {
-                  wr[j] = wh[j];
-                  ar[j] = xb[xh[j]];
-                }
-              } else {
-                const float* w = q.satw + q.nsat * ith;
-                const int* x = q.sati + q.nsat * ith;
-#pragma unroll
-                for (int j = 0; j < NSAT; ++j) wr[j] = w[j];
-#pragma unroll
-                for (int j = 0; j < NSAT; ++j) ar[j] = xb[x[j]];
-              }
-#pragma unroll
-              for (int j = 0; j < NSAT; ++j) acc = acc + wr[j] * ar[j];
-            } else {
-              const float* w = q.satw + q.nsat * ith;
-              const int* x = q.sati + q.nsat * ith;
-              for (int j = 0; j < q.nsat; ++j) acc = acc + w[j] * xb[x[j]];
+        float lim = -__builtin_inff();
+        bool found = false;
+        for (int ith = 0; ith < NTH; ++ith) {
+          const float bk = xb[ith];
+          if (!found) {
+            if (bk >= lim) {
+              lim = bk;
+              found = true;
             }
-            b = acc * facsat;
-            if (b == b && bi == 0x7fffffff) {  // passes run in ITH order: a later one wins only if greater
-              best = b;
-              bi = ith;
-            } else if (b == b && b > best) {
-              best = b;
-              bi = ith;
-            }
+          } else if (bk > lim) {
+            lim = bk;
           }
-          bv[pp] = b;
         }
-#pragma unroll
-        for (int off = 1; off < G; off <<= 1) {
-          const float ov = __shfl_xor(best, off, G);
-          const int oi = __shfl_xor(bi, off, G);
-          const bool take = oi != 0x7fffffff &&
-                            (bi == 0x7fffffff || ov > best || (ov == best && oi < bi));
-          best = take ? ov : best;
-          bi = take ? oi : bi;
-        }
-        bth0 = bi == 0x7fffffff ? __builtin_nanf("") : best;
+        bth0 = found ? lim : __builtin_nanf("");
       }
-      wave_sync();  // every lane's gathers of this band are done before X is overwritten
+      bth0w[ik0] = bth0;
       float miche;
       if (q.ssdsbm[0] == 1.f) {
         miche = 1.f;
       } else {
-        const float x = libm::tanhf(mn(kk * depth, 10.f));
+        const float x = libm::tanhf(mn(t.wn1[ik0] * depth, 10.f));
         const float tt = x * (q.ssdsbm[1] + x * (q.ssdsbm[2] + x * (q.ssdsbm[3] + x * q.ssdsbm[4])));
         miche = tt * tt;
       }
       const float coef1 = q.ssdsbr * miche;
       const float m = mx(bth0 / coef1 - 1.f, 0.f);
       const float sigk = q.sig[ik0 + 1];
-      const float sdiagiso = ((cs[1] * sigk) * cs[5]) * (m * m);
-      const float coef2 = ((cs[1] * sigk) * (1.f - cs[5])) / (coef1 * coef1);
-#pragma unroll
-      for (int pp = 0; pp < NP; ++pp) {
-        const int ith = pp * G + l;
-        if (ith < NTH) {
-          const float bb = bv[pp];
-          const float dd = sdiagiso + coef2 * libm::powf(mx(bb - coef1, 0.f), q.ssdsp);
-          const float mm = mx(__builtin_sqrtf(bb) - epsr, 0.f);
-          const float pb = (mm * mm) * 28.16f;
-          t.vdds[is0 + ith] = dd;
-          t.X[is0 + ith] = pb / twopisq;
-          t.vsds[is0 + ith] = dd * av[pp];
-        }
+      c1w[ik0] = coef1;
+      sdw[ik0] = ((cs[1] * sigk) * cs[5]) * (m * m);
+      c2w[ik0] = ((cs[1] * sigk) * (1.f - cs[5])) / (coef1 * coef1);
+    }
+    wave_sync();
+    for (int b = l; b < nspec; b += G) {
+      const int ik0 = b / NTH;
+      const float a = t.spec[b];  // X held A = SPEC until pass 1
+      if (ik0 < kb) {
+        t.vdds[b] = 0.f;
+        t.vsds[b] = 0.f * a;  // SRHS = DDIAG * A with DDIAG = 0
+        t.X[b] = 0.f;         // BTH = 0 there, so PB = MAX(-EPSR, 0)**2 * 28.16 = 0
+        continue;
       }
-      if (l == 0) bth0w[ik0] = bth0;
+      const float coef1 = c1w[ik0];
+      const float bb = iso ? bth0w[ik0] : t.X[b];
+      const float dd = sdw[ik0] + c2w[ik0] * libm::powf(mx(bb - coef1, 0.f), q.ssdsp);
+      const float mm = mx(__builtin_sqrtf(bb) - epsr, 0.f);
+      const float pb = (mm * mm) * 28.16f;
+      t.vdds[b] = dd;
+      t.X[b] = pb / twopisq;
+      t.vsds[b] = dd * a;
     }
     wave_sync();
   } else {
