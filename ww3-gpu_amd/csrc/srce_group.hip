@@ -78,16 +78,34 @@ __device__ inline int ffloor(float x) {
   return static_cast<float>(i) <= x ? i : static_cast<int>(static_cast<unsigned>(i) - 1u);
 }
 
-/// Lane K's value of v, in every lane of its group: a 16-lane DPP row (row_newbcast), or the
-/// whole wavefront (readlane).
+/// Ordered accumulation across a group, one term per lane K in K order.  bc gives every lane
+/// of a 16-lane row the value of the row's lane K % 16 (DPP row_newbcast).  A group of 16 is
+/// one row, so each step adds the group's lane K.  A wavefront group accumulates row by row:
+/// while row R's terms are added only row R's lanes hold the true running value (the other
+/// rows add their own lanes' terms, which is discarded), and row_take then gives every lane
+/// row R's running value before row R + 1 (WW_ROWS).
 template <int G, int K>
 __device__ inline float bc(float v) {
-  if constexpr (G == 16) {
-    return __builtin_bit_cast(
-        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + K, 0xF, 0xF, false));
+  static_assert(G == 16 || G == 64, "groups are a DPP row or a wavefront");
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + (K % 16), 0xF, 0xF, false));
+}
+
+/// Row R's value of v in every lane (a wavefront group); a 16-lane group's own value.
+template <int G>
+__device__ inline float row_take(float v, int r) {
+  if constexpr (G == 64) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 16 * r));
   } else {
-    static_assert(G == 64, "groups are a DPP row or a wavefront");
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), K));
+    return v;
+  }
+}
+template <int G>
+__device__ inline bool row_take(bool v, int r) {
+  if constexpr (G == 64) {
+    return __builtin_amdgcn_readlane(v ? 1 : 0, 16 * r) != 0;
+  } else {
+    return v;
   }
 }
 
@@ -102,8 +120,9 @@ __device__ inline void wave_sync() {
 #define WW_ROW16(F) F(0) F(1) F(2) F(3) F(4) F(5) F(6) F(7) F(8) F(9) F(10) F(11) F(12) F(13) F(14) F(15)
 #define WW_ROW16B(F, o) F(o + 0) F(o + 1) F(o + 2) F(o + 3) F(o + 4) F(o + 5) F(o + 6) F(o + 7) \
   F(o + 8) F(o + 9) F(o + 10) F(o + 11) F(o + 12) F(o + 13) F(o + 14) F(o + 15)
-/// F(k) for every lane k of a group (the entries k >= G are discarded at compile time).
-#define WW_ROW(F) WW_ROW16B(F, 0) WW_ROW16B(F, 16) WW_ROW16B(F, 32) WW_ROW16B(F, 48)
+/// F(k) for every lane k of a group (the entries k >= G are discarded at compile time), and
+/// S(r) after row r: a wavefront group's row_take of its running values.
+#define WW_ROWS(F, S) WW_ROW16B(F, 0) S(0) WW_ROW16B(F, 16) S(1) WW_ROW16B(F, 32) S(2) WW_ROW16B(F, 48) S(3)
 
 /// Per-point LDS layout, in floats: X (NSPEC), then the arrays that live through a sub-step
 /// (DIRF, WNF, CBETA), then one region that W3SDS4 (BTH0, the saturated-band list, the
@@ -288,7 +307,9 @@ __device__ __forceinline__ void spr4(const Grp& t, const St4Dev& q, Sc& c) {
     elsn = elsn + bc<G, k>(ps);                 \
     amax = mx(amax, bc<G, k>(aa));              \
   }
-    WW_ROW(ST)
+#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { elcs = row_take<G>(elcs, r); elsn = row_take<G>(elsn, r); amax = row_take<G>(amax, r); }
+    WW_ROWS(ST, SY)
+#undef SY
 #undef ST
   }
   const float dlw = libm::atan2f(elsn, elcs);
@@ -326,7 +347,9 @@ __device__ __forceinline__ void spr4(const Grp& t, const St4Dev& q, Sc& c) {
     emeanws = emeanws + bc<G, k>(e2);           \
     fmeanws = fmeanws + bc<G, k>(tfw);          \
   }
-    WW_ROW(ST)
+#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { emean = row_take<G>(emean, r); fmean = row_take<G>(fmean, r); fmean1 = row_take<G>(fmean1, r); wnm = row_take<G>(wnm, r); emeanws = row_take<G>(emeanws, r); fmeanws = row_take<G>(fmeanws, r); }
+    WW_ROWS(ST, SY)
+#undef SY
 #undef ST
     if (base + G >= nk) {
       elast = __shfl(e, nk - 1 - base, G);
@@ -408,7 +431,9 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
     uorb = uorb + bc<G, k>(tu);       \
     aorb = aorb + bc<G, k>(ta);       \
   }
-    WW_ROW(ST)
+#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { uorb = row_take<G>(uorb, r); aorb = row_take<G>(aorb, r); }
+    WW_ROWS(ST, SY)
+#undef SY
 #undef ST
   }
   uorb = 2.f * __builtin_sqrtf(uorb);
@@ -548,7 +573,9 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
     ssx = ssx + bc<G, k>(pcs);                      \
     ssy = ssy + bc<G, k>(pss);                      \
   }
-        WW_ROW(ST)
+#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { snx = row_take<G>(snx, r); sny = row_take<G>(sny, r); ssx = row_take<G>(ssx, r); ssy = row_take<G>(ssy, r); }
+        WW_ROWS(ST, SY)
+#undef SY
 #undef ST
       }
     }
@@ -599,7 +626,9 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
     const int cnt = nth - base;
 #define ST(k) if constexpr ((k) < G) \
   if (k < cnt) temp = temp + bc<G, k>(term);
-    WW_ROW(ST)
+#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { temp = row_take<G>(temp, r); }
+    WW_ROWS(ST, SY)
+#undef SY
 #undef ST
   }
   const float levtail0 = const0 * temp;
@@ -992,15 +1021,17 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
             lim = bk;                                                             \
           }                                                                       \
         }                                                                         \
-        if (ith == nth - 1 && l == 0) {                                           \
+        if (ith == nth - 1 && l == ((k) & 48)) {                                  \
           const float kk = t.wn1[ik0];                                            \
           const float facsat = (q.sig[ik0 + 1] * ((kk * kk) * kk)) * q.dth;       \
           bth0w[ik0] = iso ? asum * facsat : (found ? lim : __builtin_nanf(""));  \
         }                                                                         \
       }                                                                           \
     }
-      WW_ROW(ST)
-  #undef ST
+#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { lim = row_take<G>(lim, r); found = row_take<G>(found, r); asum = row_take<G>(asum, r); }
+      WW_ROWS(ST, SY)
+  #undef SY
+#undef ST
     }
     for (int ik0 = l; ik0 < ik1 - 1 && ik0 < nk; ik0 += G) bth0w[ik0] = 0.f;
     wave_sync();
@@ -1080,7 +1111,9 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
     }                                                        \
     if (f.flogrd58 && ikk >= ik1) wc1 = wc1 + c4 * mf;       \
   }
-      WW_ROW(ST)
+#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { wc0 = row_take<G>(wc0, r); wc1 = row_take<G>(wc1, r); wc3 = row_take<G>(wc3, r); }
+      WW_ROWS(ST, SY)
+#undef SY
 #undef ST
     }
     if (l == 0) {
@@ -1467,7 +1500,9 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
       dtl = mn(dtl, afac / mx(mn(vd, 0.f) * afac + 1.f, 1.E-10f));
     }
 #define ST(k) if constexpr ((k) < G) dt = mn(dt, bc<G, k>(dtl));
-    WW_ROW(ST)
+#define SY(r) if constexpr (G == 64) { dt = row_take<G>(dt, r); }
+    WW_ROWS(ST, SY)
+#undef SY
 #undef ST
   }
   dt = mx(dt, 0.5f);
@@ -1564,7 +1599,9 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
     phibbl = phibbl - bc<G, k>(tpb);               \
     wc2 = wc2 + bc<G, k>(twc);                     \
   }
-    WW_ROW(ST)
+#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { phiaw = row_take<G>(phiaw, r); phibbl = row_take<G>(phibbl, r); wc2 = row_take<G>(wc2, r); }
+    WW_ROWS(ST, SY)
+#undef SY
 #undef ST
   }
   wave_sync();
@@ -1751,7 +1788,9 @@ __global__ __launch_bounds__(64) void finish_kernel(State s, St4Dev q, SrceDev p
     mwx = bc<G, k>(tx) + mwx;             \
     mwy = bc<G, k>(ty) + mwy;             \
   }
-    WW_ROW(ST)
+#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { efinish = row_take<G>(efinish, r); mwx = row_take<G>(mwx, r); mwy = row_take<G>(mwy, r); }
+    WW_ROWS(ST, SY)
+#undef SY
 #undef ST
   }
   wave_sync();
@@ -1807,7 +1846,9 @@ __global__ __launch_bounds__(64) void finish_kernel(State s, St4Dev q, SrceDev p
     ti0 = ti0 - bc<G, k>(d0);         \
     ti1 = ti1 - bc<G, k>(d1);         \
   }
-      WW_ROW(ST)
+#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { phice = row_take<G>(phice, r); ti0 = row_take<G>(ti0, r); ti1 = row_take<G>(ti1, r); }
+      WW_ROWS(ST, SY)
+#undef SY
 #undef ST
     }
     c.v[S_PHICE] = (((-1.f * kDwat) * kGrav) * phice) / dtg;
