@@ -4,7 +4,8 @@
 // Test harness, not part of the library.
 //
 // Usage: devlibm_host_check <fn> [threads] [samples]
-//   one-argument functions (expf logf log10f tanhf sinf cosf): every float input;
+//   one-argument functions (expf logf log10f tanhf sinf cosf, pow025f = powf(x, 0.25)):
+//   every float input;
 //   two-argument functions (powf atan2f): `samples` random pairs plus structured ones.
 // Prints one line: fn, inputs, dev!=oracle, dev!=cr, dev!=glibc, max ulp vs cr, vs glibc.
 // Exit status 1 if the device and oracle definitions ever differ (for expf: if the device
@@ -53,6 +54,7 @@ Eval eval1_fast(const std::string& fn, float x) {
   if (fn == "logf") return {L::logf(x), wl_logf(x), 0, 0};
   if (fn == "log10f") return {L::log10f(x), wl_log10f(x), 0, 0};
   if (fn == "tanhf") return {L::tanhf(x), wl_tanhf(x), 0, 0};
+  if (fn == "pow025f") return {L::powf(x, 0.25f), wl_powf(x, 0.25f), 0, 0};
   float s, c, so, co;
   L::sincosf(x, &s, &c);
   wl_sincosf(x, &so, &co);
@@ -66,6 +68,7 @@ Eval eval1(const std::string& fn, float x) {
   if (fn == "logf") return {L::logf(x), wl_logf(x), (float)logl(x), ::logf(x)};
   if (fn == "log10f") return {L::log10f(x), wl_log10f(x), (float)log10l(x), ::log10f(x)};
   if (fn == "tanhf") return {L::tanhf(x), wl_tanhf(x), (float)tanhl(x), ::tanhf(x)};
+  if (fn == "pow025f") return {L::powf(x, 0.25f), wl_powf(x, 0.25f), (float)powl(x, 0.25L), ::powf(x, 0.25f)};
   if (fn == "sinf") {
     float s, c, so, co;
     L::sincosf(x, &s, &c);

@@ -52,7 +52,7 @@ def test_expf_equals_glibc_everywhere(checker):
     assert rc == 0 and v["dev_ne_glibc"] == 0
 
 
-@pytest.mark.parametrize("fn", ["logf", "log10f", "tanhf", "sinf", "cosf"])
+@pytest.mark.parametrize("fn", ["logf", "log10f", "tanhf", "sinf", "cosf", "pow025f"])
 def test_unary_all_inputs(checker, fn):
     # device vs oracle on every EVERY-th input; vs the references on every 61st of those
     rc, v = run(checker, fn, samples=61, every=EVERY)

@@ -144,8 +144,9 @@ struct Lds {
     pk = un;
     const int tail = (tail7 ? 7 : 3) * kTailStride;
     small = un + (tail > 4 * nk + 1 ? tail : 4 * nk + 1);
-    // X and W of the four points start 8 banks apart
-    stride = ((nk * nth + small + 31) / 32) * 32 + 8;
+    // X and W of the four points start 16 banks apart: the 16 consecutive floats that each
+    // 16-lane row reads or writes in one access never share a bank with another row's
+    stride = ((nk * nth + small + 31) / 32) * 32 + 16;
   }
 };
 
@@ -537,7 +538,9 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
             const float zlog = zcn + zarg;
             if (zlog < 0.f) {
               const float zl2 = zlog * zlog;
-              dv = cnst * libm::expf(zlog) * (zl2 * zl2) * ucn * ucn * libm::powf(coswind, q.ssinthp);
+              // COSWIND > 0.01 here, where POWF(COSWIND, 2) is COSWIND*COSWIND
+              const float pw = q.ssinthp == 2.f ? coswind * coswind : libm::powf(coswind, q.ssinthp);
+              dv = cnst * libm::expf(zlog) * (zl2 * zl2) * ucn * ucn * pw;
               lw = 1;
             } else {
               dv = 0.f;
@@ -951,7 +954,8 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
       }
       const float coef1 = c1w[ik0];
       const float bb = iso ? bth0w[ik0] : t.X[b];
-      const float dd = sdw[ik0] + c2w[ik0] * libm::powf(mx(bb - coef1, 0.f), q.ssdsp);
+      const float xp = mx(bb - coef1, 0.f);  // >= 0 or +inf, where POWF(XP, 2) is XP*XP
+      const float dd = sdw[ik0] + c2w[ik0] * (q.ssdsp == 2.f ? xp * xp : libm::powf(xp, q.ssdsp));
       const float mm = mx(__builtin_sqrtf(bb) - epsr, 0.f);
       const float pb = (mm * mm) * 28.16f;
       t.vdds[b] = dd;
@@ -1062,7 +1066,8 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
       const float sdiagiso = ((cs[1] * sigk) * cs[5]) * (m * m);
       const float coef2 = ((cs[1] * sigk) * (1.f - cs[5])) / (coef1 * coef1);
       const float bb = iso ? bth0 : t.vsds[b];
-      const float dd = sdiagiso + coef2 * libm::powf(mx(bb - coef1, 0.f), q.ssdsp);
+      const float xp = mx(bb - coef1, 0.f);  // >= 0 or +inf, where POWF(XP, 2) is XP*XP
+      const float dd = sdiagiso + coef2 * (q.ssdsp == 2.f ? xp * xp : libm::powf(xp, q.ssdsp));
       const float mm = mx(__builtin_sqrtf(bb) - epsr, 0.f);
       const float pb = (mm * mm) * 28.16f;
       t.vdds[b] = dd;
