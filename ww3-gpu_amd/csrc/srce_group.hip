@@ -719,79 +719,95 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
 //
 // RENEW(IS) = sum over saturated IK2 <= IK - DIKCUMUL, in IK2 order, of
 // V = sum over ITH2 in order of CUMULW(IS2, IS) * BRLAMBDA(IS2).  For one (IK, IK2), the NTH
-// values of V are a Toeplitz product: CUMULW depends on |ITH2 - ITH| only.  Each lane takes
-// output bands IK = base + l and base + 31 - l (the two together hold about the same number
-// of (IK, IK2) pairs for every lane), keeps the mirrored CUMULW row of a pair in registers and
-// runs the NTH x NTH products with every index fixed at compile time: per pair, 2*NTH loads
-// and NTH*NTH multiply-add pairs on NTH independent accumulators.  RENEW goes to the point's
-// RNW array.  X holds BRLAMBDA; W's sat list holds the saturated bands in order.
+// values of V are a Toeplitz product: CUMULW depends on |ITH2 - ITH| only.  Sixteen lanes of
+// a group take band pairs IK = base + p and base + 31 - p, p = 0..15 (the two together hold
+// about the same number of (IK, IK2) pairs for every lane); a group of G lanes splits each
+// pair's NTH output directions into G/16 slices of NI = 16*NTH/G.  A lane keeps the window
+// of the mirrored CUMULW row that its slice needs in registers and runs the NTH x NI
+// products with every register index fixed at compile time: per pair, NTH + NI - 1 row
+// loads, NTH loads of BRLAMBDA and NTH*NI multiply-add pairs on NI independent accumulators.
+// RENEW goes to the point's RNW array.  X holds BRLAMBDA; W's sat list holds the saturated
+// bands in order.
 // ------------------------------------------------------------------------------------
 template <int G, int NTH>
 __device__ __forceinline__ void cumul_aniso(const Grp& t, const St4Dev& q, int dikcumul) {
   constexpr int MS = kMirStride(NTH);
-  const int nk = q.nk, l = t.l;
+  constexpr int DS = G / 16;     // lanes per band pair
+  constexpr int NI = NTH / DS;   // output directions per lane
+  constexpr int NW = NTH + NI - 1;  // the CUMULW window of a slice
+  static_assert(NTH % DS == 0, "the direction slices must be equal");
+  const int nk = q.nk;
+  const int pi = t.l % 16, i0 = (t.l / 16) * NI;
   const int* sat = reinterpret_cast<const int*>(t.W + t.o.sat);
   const int nsat = sat[0];
-  for (int base = 0; base < nk; base += 2 * G) {
-    const int ika = base + l, ikb = base + 2 * G - 1 - l;  // 0-based output bands
+  for (int base = 0; base < nk; base += 32) {
+    const int ika = base + pi, ikb = base + 31 - pi;  // 0-based output bands
     const int lima = ika + 1 - dikcumul, limb = ikb + 1 - dikcumul;
     int ca = 0, cb = 0;
     if (ika < nk && lima > 0)
       while (ca < nsat && sat[1 + ca] <= lima) ++ca;
     if (ikb < nk && limb > 0)
       while (cb < nsat && sat[1 + cb] <= limb) ++cb;
-    float rn[NTH];
+    float rn[NI];
 #pragma unroll
-    for (int i = 0; i < NTH; ++i) rn[i] = 0.f;
+    for (int i = 0; i < NI; ++i) rn[i] = 0.f;
     const int total = ca + cb;
     for (int m = 0; m < total; ++m) {
       if (m == ca) {  // band a is complete
         if (ika < nk) {
 #pragma unroll
-          for (int i = 0; i < NTH; ++i) t.rnw[ika * NTH + i] = rn[i];
+          for (int i = 0; i < NI; ++i) t.rnw[ika * NTH + i0 + i] = rn[i];
         }
 #pragma unroll
-        for (int i = 0; i < NTH; ++i) rn[i] = 0.f;
+        for (int i = 0; i < NI; ++i) rn[i] = 0.f;
       }
       const bool ina = m < ca;
       const int ik0 = ina ? ika : ikb;
       const int ik2 = sat[1 + (ina ? m : m - ca)];
-      const float4* kr4 =
-          reinterpret_cast<const float4*>(q.cumulm + (static_cast<size_t>(ik0) * nk + (ik2 - 1)) * MS);
-      float km[MS];
+      const float* row = q.cumulm + (static_cast<size_t>(ik0) * nk + (ik2 - 1)) * MS;
+      // kw[w] = CUMULW row entry NTH - NI - i0 + w: output i0 + i takes input j from
+      // entry j - (i0 + i) + NTH - 1 = kw[j - i + NI - 1]
+      float kw[NW];
+      if constexpr (DS == 1) {
+        const float4* kr4 = reinterpret_cast<const float4*>(row);
 #pragma unroll
-      for (int d = 0; d < MS / 4; ++d) {
-        const float4 v = kr4[d];
-        km[4 * d] = v.x;
-        km[4 * d + 1] = v.y;
-        km[4 * d + 2] = v.z;
-        km[4 * d + 3] = v.w;
+        for (int d = 0; d < MS / 4; ++d) {
+          const float4 v = kr4[d];
+          if (4 * d < NW) kw[4 * d] = v.x;
+          if (4 * d + 1 < NW) kw[4 * d + 1] = v.y;
+          if (4 * d + 2 < NW) kw[4 * d + 2] = v.z;
+          if (4 * d + 3 < NW) kw[4 * d + 3] = v.w;
+        }
+      } else {
+        const float* kr = row + (NTH - NI - i0);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) kw[w] = kr[w];
       }
       const float* bl = t.X + (ik2 - 1) * NTH;
-      float acc[NTH];
+      float acc[NI];
 #pragma unroll
-      for (int i = 0; i < NTH; ++i) acc[i] = 0.f;
+      for (int i = 0; i < NI; ++i) acc[i] = 0.f;
 #pragma unroll
       for (int j = 0; j < NTH; ++j) {
         const float b = bl[j];
 #pragma unroll
-        for (int i = 0; i < NTH; ++i) acc[i] = acc[i] + km[j - i + NTH - 1] * b;
+        for (int i = 0; i < NI; ++i) acc[i] = acc[i] + kw[j - i + NI - 1] * b;
       }
 #pragma unroll
-      for (int i = 0; i < NTH; ++i) rn[i] = rn[i] + acc[i];
+      for (int i = 0; i < NI; ++i) rn[i] = rn[i] + acc[i];
     }
     if (ca == total) {  // no switch: rn is band a's, band b has no pair
       if (ika < nk) {
 #pragma unroll
-        for (int i = 0; i < NTH; ++i) t.rnw[ika * NTH + i] = rn[i];
+        for (int i = 0; i < NI; ++i) t.rnw[ika * NTH + i0 + i] = rn[i];
       }
       if (ikb < nk) {
 #pragma unroll
-        for (int i = 0; i < NTH; ++i) t.rnw[ikb * NTH + i] = 0.f;
+        for (int i = 0; i < NI; ++i) t.rnw[ikb * NTH + i0 + i] = 0.f;
       }
     } else if (ikb < nk) {
 #pragma unroll
-      for (int i = 0; i < NTH; ++i) t.rnw[ikb * NTH + i] = rn[i];
+      for (int i = 0; i < NI; ++i) t.rnw[ikb * NTH + i0 + i] = rn[i];
     }
   }
 }
