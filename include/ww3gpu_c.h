@@ -267,6 +267,12 @@ int ww3gpu_srce_substeps(int igrid);
  * each sub-step, summed over its sub-steps), or -1. */
 long ww3gpu_srce_point_substeps(int igrid);
 
+/* Test control, with no counterpart in the reference: the lanes per point of the W3SRCE
+ * sub-step kernel for every later call of every grid, 16, 32 or 64, or 0 (the default: the
+ * width follows each sub-step's active-point count).  Results do not depend on it.  Returns
+ * OK or BAD_SHAPE. */
+int ww3gpu_srce_group_lanes(int lanes);
+
 /* Per-kernel timing of grid igrid's W3SRCE calls.  enable = 1 starts it (and clears the
  * timers), 0 stops it.  While on, every launch of a call is bracketed by HIP events on the
  * call's stream.  Returns OK or NOT_INITIALISED. */

@@ -197,3 +197,10 @@ def test_oracle_energy_budget_closes():
         ref = (phiaw + (e0 - e1) - phibbl) * dg / 300.0
         got = float(out["psc"][p, P("PHIOC")])
         assert np.isclose(got, ref, rtol=2e-3, atol=1e-4 * max(1.0, abs(ref))), (p, got, ref)
+
+
+def test_group_lanes_control():
+    L = capi.lib()
+    for bad in (-1, 8, 48, 128):
+        assert L.ww3gpu_srce_group_lanes(bad) == capi.Status.BAD_SHAPE
+    assert L.ww3gpu_srce_group_lanes(0) == capi.Status.OK

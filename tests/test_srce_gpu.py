@@ -115,6 +115,21 @@ def test_variant(rt, name):
     _assert_same(got, ref, name)
 
 
+@pytest.mark.parametrize("lanes", [16, 32, 64])
+@pytest.mark.parametrize("nk,nth", [(32, 36), (25, 24), (20, 12)])
+def test_group_widths(rt, lanes, nk, nth):
+    """Every sub-step at one group width (16, 32 or 64 lanes per point; the compiled NTH = 36,
+    24 and the generic path): the same bits as the oracle."""
+    L = capi.lib()
+    c = _variant(nk=nk, nth=nth, npts=96, seed=31 + lanes)
+    assert L.ww3gpu_srce_group_lanes(lanes) == capi.Status.OK
+    try:
+        got, ref, _ = _run(c)
+    finally:
+        L.ww3gpu_srce_group_lanes(0)
+    _assert_same(got, ref, f"{lanes} lanes, nk {nk} nth {nth}")
+
+
 def test_calm_and_extreme_points(rt):
     """U10 = 0 (u* = 0: FHIGH = +Inf on the first sub-step), 35 m/s, a zero spectrum, very
     shallow water."""

@@ -1219,4 +1219,10 @@ long ww3gpu_srce_point_substeps(int igrid) {
   return g == nullptr ? -1 : g->point_substeps;
 }
 
+int ww3gpu_srce_group_lanes(int lanes) {
+  if (lanes != 0 && lanes != 16 && lanes != 32 && lanes != 64) return WW_KOKKOS_ERR_BAD_SHAPE;
+  ww3gpu::srce::set_group_lanes(lanes);
+  return WW_KOKKOS_OK;
+}
+
 }  // extern "C"

@@ -40,6 +40,8 @@
 // whitecap sums run after the cumulative term, which does not touch their inputs.  The
 // time-step limit's MIN over bins is formed in lane order: its operands are never NaN or
 // -0 (each is AFAC / MAX(., 1E-10) >= +0), so the order cannot change the result.
+#include <atomic>
+
 #include "devlibm.hpp"
 #include "srce.hpp"
 
@@ -80,22 +82,25 @@ __device__ inline int ffloor(float x) {
 
 /// Ordered accumulation across a group, one term per lane K in K order.  bc gives every lane
 /// of a 16-lane row the value of the row's lane K % 16 (DPP row_newbcast).  A group of 16 is
-/// one row, so each step adds the group's lane K.  A wavefront group accumulates row by row:
-/// while row R's terms are added only row R's lanes hold the true running value (the other
-/// rows add their own lanes' terms, which is discarded), and row_take then gives every lane
-/// row R's running value before row R + 1 (WW_ROWS).
+/// one row, so each step adds the group's lane K.  A group of 32 or 64 lanes accumulates row
+/// by row: while row R's terms are added only row R's lanes hold the true running value (the
+/// other rows add their own lanes' terms, which is discarded), and row_take then gives every
+/// lane of the group row R's running value before row R + 1 (WW_ROWS).
 template <int G, int K>
 __device__ inline float bc(float v) {
-  static_assert(G == 16 || G == 64, "groups are a DPP row or a wavefront");
+  static_assert(G == 16 || G == 32 || G == 64, "groups are one, two or four DPP rows");
   return __builtin_bit_cast(
       float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + (K % 16), 0xF, 0xF, false));
 }
 
-/// Row R's value of v in every lane (a wavefront group); a 16-lane group's own value.
+/// Row R's value of v in every lane of the group (a 32- or 64-lane group); a 16-lane group's
+/// own value.
 template <int G>
 __device__ inline float row_take(float v, int r) {
   if constexpr (G == 64) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 16 * r));
+  } else if constexpr (G == 32) {
+    return __shfl(v, 16 * r, 32);
   } else {
     return v;
   }
@@ -104,6 +109,8 @@ template <int G>
 __device__ inline bool row_take(bool v, int r) {
   if constexpr (G == 64) {
     return __builtin_amdgcn_readlane(v ? 1 : 0, 16 * r) != 0;
+  } else if constexpr (G == 32) {
+    return __shfl(v ? 1 : 0, 16 * r, 32) != 0;
   } else {
     return v;
   }
@@ -121,7 +128,7 @@ __device__ inline void wave_sync() {
 #define WW_ROW16B(F, o) F(o + 0) F(o + 1) F(o + 2) F(o + 3) F(o + 4) F(o + 5) F(o + 6) F(o + 7) \
   F(o + 8) F(o + 9) F(o + 10) F(o + 11) F(o + 12) F(o + 13) F(o + 14) F(o + 15)
 /// F(k) for every lane k of a group (the entries k >= G are discarded at compile time), and
-/// S(r) after row r: a wavefront group's row_take of its running values.
+/// S(r) after row r: the row_take of the running values in a group wider than a row.
 #define WW_ROWS(F, S) WW_ROW16B(F, 0) S(0) WW_ROW16B(F, 16) S(1) WW_ROW16B(F, 32) S(2) WW_ROW16B(F, 48) S(3)
 
 /// Per-point LDS layout, in floats: X (NSPEC), then the arrays that live through a sub-step
@@ -308,7 +315,7 @@ __device__ __forceinline__ void spr4(const Grp& t, const St4Dev& q, Sc& c) {
     elsn = elsn + bc<G, k>(ps);                 \
     amax = mx(amax, bc<G, k>(aa));              \
   }
-#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { elcs = row_take<G>(elcs, r); elsn = row_take<G>(elsn, r); amax = row_take<G>(amax, r); }
+#define SY(r) if constexpr (G > 16 && 16 * (r) < G) if (16 * (r) < cnt) { elcs = row_take<G>(elcs, r); elsn = row_take<G>(elsn, r); amax = row_take<G>(amax, r); }
     WW_ROWS(ST, SY)
 #undef SY
 #undef ST
@@ -348,7 +355,7 @@ __device__ __forceinline__ void spr4(const Grp& t, const St4Dev& q, Sc& c) {
     emeanws = emeanws + bc<G, k>(e2);           \
     fmeanws = fmeanws + bc<G, k>(tfw);          \
   }
-#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { emean = row_take<G>(emean, r); fmean = row_take<G>(fmean, r); fmean1 = row_take<G>(fmean1, r); wnm = row_take<G>(wnm, r); emeanws = row_take<G>(emeanws, r); fmeanws = row_take<G>(fmeanws, r); }
+#define SY(r) if constexpr (G > 16 && 16 * (r) < G) if (16 * (r) < cnt) { emean = row_take<G>(emean, r); fmean = row_take<G>(fmean, r); fmean1 = row_take<G>(fmean1, r); wnm = row_take<G>(wnm, r); emeanws = row_take<G>(emeanws, r); fmeanws = row_take<G>(fmeanws, r); }
     WW_ROWS(ST, SY)
 #undef SY
 #undef ST
@@ -432,7 +439,7 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
     uorb = uorb + bc<G, k>(tu);       \
     aorb = aorb + bc<G, k>(ta);       \
   }
-#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { uorb = row_take<G>(uorb, r); aorb = row_take<G>(aorb, r); }
+#define SY(r) if constexpr (G > 16 && 16 * (r) < G) if (16 * (r) < cnt) { uorb = row_take<G>(uorb, r); aorb = row_take<G>(aorb, r); }
     WW_ROWS(ST, SY)
 #undef SY
 #undef ST
@@ -576,7 +583,7 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
     ssx = ssx + bc<G, k>(pcs);                      \
     ssy = ssy + bc<G, k>(pss);                      \
   }
-#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { snx = row_take<G>(snx, r); sny = row_take<G>(sny, r); ssx = row_take<G>(ssx, r); ssy = row_take<G>(ssy, r); }
+#define SY(r) if constexpr (G > 16 && 16 * (r) < G) if (16 * (r) < cnt) { snx = row_take<G>(snx, r); sny = row_take<G>(sny, r); ssx = row_take<G>(ssx, r); ssy = row_take<G>(ssy, r); }
         WW_ROWS(ST, SY)
 #undef SY
 #undef ST
@@ -629,7 +636,7 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
     const int cnt = nth - base;
 #define ST(k) if constexpr ((k) < G) \
   if (k < cnt) temp = temp + bc<G, k>(term);
-#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { temp = row_take<G>(temp, r); }
+#define SY(r) if constexpr (G > 16 && 16 * (r) < G) if (16 * (r) < cnt) { temp = row_take<G>(temp, r); }
     WW_ROWS(ST, SY)
 #undef SY
 #undef ST
@@ -1048,7 +1055,7 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
         }                                                                         \
       }                                                                           \
     }
-#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { lim = row_take<G>(lim, r); found = row_take<G>(found, r); asum = row_take<G>(asum, r); }
+#define SY(r) if constexpr (G > 16 && 16 * (r) < G) if (16 * (r) < cnt) { lim = row_take<G>(lim, r); found = row_take<G>(found, r); asum = row_take<G>(asum, r); }
       WW_ROWS(ST, SY)
   #undef SY
 #undef ST
@@ -1132,7 +1139,7 @@ __device__ __forceinline__ void sds4(const Grp& t, const St4Dev& q, const StepFl
     }                                                        \
     if (f.flogrd58 && ikk >= ik1) wc1 = wc1 + c4 * mf;       \
   }
-#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { wc0 = row_take<G>(wc0, r); wc1 = row_take<G>(wc1, r); wc3 = row_take<G>(wc3, r); }
+#define SY(r) if constexpr (G > 16 && 16 * (r) < G) if (16 * (r) < cnt) { wc0 = row_take<G>(wc0, r); wc1 = row_take<G>(wc1, r); wc3 = row_take<G>(wc3, r); }
       WW_ROWS(ST, SY)
 #undef SY
 #undef ST
@@ -1521,7 +1528,7 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
       dtl = mn(dtl, afac / mx(mn(vd, 0.f) * afac + 1.f, 1.E-10f));
     }
 #define ST(k) if constexpr ((k) < G) dt = mn(dt, bc<G, k>(dtl));
-#define SY(r) if constexpr (G == 64) { dt = row_take<G>(dt, r); }
+#define SY(r) if constexpr (G > 16 && 16 * (r) < G) { dt = row_take<G>(dt, r); }
     WW_ROWS(ST, SY)
 #undef SY
 #undef ST
@@ -1620,7 +1627,7 @@ __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp,
     phibbl = phibbl - bc<G, k>(tpb);               \
     wc2 = wc2 + bc<G, k>(twc);                     \
   }
-#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { phiaw = row_take<G>(phiaw, r); phibbl = row_take<G>(phibbl, r); wc2 = row_take<G>(wc2, r); }
+#define SY(r) if constexpr (G > 16 && 16 * (r) < G) if (16 * (r) < cnt) { phiaw = row_take<G>(phiaw, r); phibbl = row_take<G>(phibbl, r); wc2 = row_take<G>(wc2, r); }
     WW_ROWS(ST, SY)
 #undef SY
 #undef ST
@@ -1809,7 +1816,7 @@ __global__ __launch_bounds__(64) void finish_kernel(State s, St4Dev q, SrceDev p
     mwx = bc<G, k>(tx) + mwx;             \
     mwy = bc<G, k>(ty) + mwy;             \
   }
-#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { efinish = row_take<G>(efinish, r); mwx = row_take<G>(mwx, r); mwy = row_take<G>(mwy, r); }
+#define SY(r) if constexpr (G > 16 && 16 * (r) < G) if (16 * (r) < cnt) { efinish = row_take<G>(efinish, r); mwx = row_take<G>(mwx, r); mwy = row_take<G>(mwy, r); }
     WW_ROWS(ST, SY)
 #undef SY
 #undef ST
@@ -1867,7 +1874,7 @@ __global__ __launch_bounds__(64) void finish_kernel(State s, St4Dev q, SrceDev p
     ti0 = ti0 - bc<G, k>(d0);         \
     ti1 = ti1 - bc<G, k>(d1);         \
   }
-#define SY(r) if constexpr (G == 64) if (16 * (r) < cnt) { phice = row_take<G>(phice, r); ti0 = row_take<G>(ti0, r); ti1 = row_take<G>(ti1, r); }
+#define SY(r) if constexpr (G > 16 && 16 * (r) < G) if (16 * (r) < cnt) { phice = row_take<G>(phice, r); ti0 = row_take<G>(ti0, r); ti1 = row_take<G>(ti1, r); }
       WW_ROWS(ST, SY)
 #undef SY
 #undef ST
@@ -1975,12 +1982,22 @@ hipError_t launch_step(const State& s, const St4Dev& q, const SrceDev& p, const 
   return hipGetLastError();
 }
 
+namespace {
+std::atomic<int> g_group_lanes{0};  // set_group_lanes
+}  // namespace
+
 hipError_t k_step(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, int iter, int n,
                   hipStream_t st) {
-  // Many points: a 16-lane row per point (four per wavefront) for throughput.  Few points
-  // (the late sub-steps): a wavefront per point, which shortens each point's sub-step.
-  return n > kWideBelow ? launch_step<16>(s, q, p, f, iter, n, st) : launch_step<64>(s, q, p, f, iter, n, st);
+  // Many points: a 16-lane row per point (four per wavefront) for throughput.  Fewer: two
+  // rows per point, then (the late sub-steps) a wavefront per point, which shorten each
+  // point's sub-step while every wavefront still fits the GPU in one round.
+  const int lanes = g_group_lanes.load(std::memory_order_relaxed);
+  if (lanes == 16 || (lanes == 0 && n > kHalfBelow)) return launch_step<16>(s, q, p, f, iter, n, st);
+  if (lanes == 32 || (lanes == 0 && n > kWideBelow)) return launch_step<32>(s, q, p, f, iter, n, st);
+  return launch_step<64>(s, q, p, f, iter, n, st);
 }
+
+void set_group_lanes(int lanes) { g_group_lanes.store(lanes, std::memory_order_relaxed); }
 
 hipError_t k_routine(const St4Call& io, const St4Dev& q, hipStream_t st) {
   if (io.npts <= 0) return hipSuccess;
