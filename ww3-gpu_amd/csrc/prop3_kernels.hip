@@ -255,6 +255,103 @@ __global__ __launch_bounds__(256) void avg_kernel(XyGrid g, XyState s, int itloc
   vq[k] = v;
 }
 
+// The averaging through LDS.  A workgroup takes kAvgT consecutive plane entries (targets) of
+// one component; every source a target can receive from lies within NY+1 cells, so the tile's
+// sources are the kAvgT + 2 NY + 2 cells around it.  Each source is read once, coalesced, and
+// its five store values (((P * A) * Q) / D, the factor sets of avg_fold) are formed once into
+// LDS with its store word; the targets then fold from LDS in the order avg_fold uses, adding
+// the same values.  A transparent source (MAPTRN) keeps its A (kTrnBit).  The global closure's
+// far-side folds (first and last NY cells) read global memory through avg_fold.
+constexpr int kAvgT = 1024;
+constexpr int kAvgBS = 256;
+constexpr uint64_t kTrnBit = 1ULL << 63;
+
+__device__ __forceinline__ int dir_offset(int d, int ny) {
+  return d == 0 ? 0 : (d == 1 ? 1 : d == 2 ? -1 : d == 3 ? ny : d == 4 ? -ny : d == 5 ? ny + 1 : d == 6 ? ny - 1 : d == 7 ? -ny + 1 : -ny - 1);
+}
+
+__global__ __launch_bounds__(kAvgBS) void avg_tiled_kernel(XyGrid g, XyState s, int itloc) {
+  extern __shared__ uint64_t avg_lds[];
+  const int c = blockIdx.y;
+  if (itloc > s.comps[c].ntloc) return;
+  const int ny = g.ny, nl = kAvgT + 2 * ny + 2;  // nl is even: the store words stay aligned
+  uint64_t* tws = avg_lds;                        // [nl]
+  float* tm = reinterpret_cast<float*>(avg_lds + nl);  // [5][nl]
+  const int k0 = static_cast<int>(blockIdx.x) * kAvgT;
+  const int s0 = k0 + 1 - ny - ny - 1;  // the first source cell: first target - (NY + 1)
+  const float* aq = s.q[(itloc - 1) & 1] + static_cast<size_t>(c) * g.nq;
+  float* vq = s.q[itloc & 1] + static_cast<size_t>(c) * g.nq;
+  const size_t rb = static_cast<size_t>(c) * (g.nxy + 1);
+  for (int i = static_cast<int>(threadIdx.x); i < nl; i += kAvgBS) {
+    const int src = s0 + i;
+    uint64_t w = 0;
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f, e4 = 0.f;
+    if (src >= 1 && src <= g.nxy) {
+      const float a = aq[src + ny - 1];
+      if (g.maptrn[src - 1] != 0) {
+        w = kTrnBit;
+        e0 = a;
+      } else {
+        w = s.tw[rb + src];
+        const Rec r = s.rec[rb + src];
+        e0 = (((3.f - r.rd2) - r.rd4) * a) / 3.f;
+        e1 = ((r.rd2 * a) * r.rd1) / 6.f;
+        e2 = (((1.f - r.rd1) * a) * r.rd2) / 6.f;
+        e3 = ((r.rd4 * a) * r.rd3) / 6.f;
+        e4 = (((1.f - r.rd3) * a) * r.rd4) / 6.f;
+      }
+    }
+    tws[i] = w;
+    tm[i] = e0;
+    tm[nl + i] = e1;
+    tm[2 * nl + i] = e2;
+    tm[3 * nl + i] = e3;
+    tm[4 * nl + i] = e4;
+  }
+  __syncthreads();
+  for (int j = static_cast<int>(threadIdx.x); j < kAvgT; j += kAvgBS) {
+    const int k = k0 + j;
+    if (k >= g.nq) break;
+    const int t = k + 1 - ny;
+    uint64_t ord = g.order[k];
+    float v = 0.f;
+    for (;;) {
+      const int d = static_cast<int>(ord & 0xFu);
+      if (d == 0xF) break;
+      ord >>= 4;
+      const int i = t - dir_offset(d, ny) - s0;
+      const uint64_t w = tws[i];
+      if (w & kTrnBit) {  // :1157-1158, :1192-1193 (listed at its own cell only)
+        v = tm[i];
+        continue;
+      }
+      const bool coast = (w >> kNearCoastBit) & 1ULL;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        if (static_cast<int>((w >> (4 * q)) & 0xFu) != d) continue;
+        const int kk = q == 0 ? 0 : (coast ? (q + 1) / 2 : (q - 1) % 4 + 1);
+        v = v + tm[kk * nl + i];
+      }
+    }
+    if (t >= 1 && t <= g.nxy) {
+      if (g.mapsta[t - 1] == 2) v = aq[k];  // :1280-1282
+      if (g.global) {                       // :1286-1291
+        if (t <= g.ny) v = v + avg_fold(g, s, c, aq, g.nxy + t);
+        if (t > g.nxy - g.ny) v = v + avg_fold(g, s, c, aq, t - g.nxy);
+      }
+      if (g.cellsea[t] > 0) v = v * g.gsqrt[t - 1];  // :1301-1306
+    }
+    vq[k] = v;
+  }
+}
+
+/// LDS bytes of an averaging tile (0: the tile does not fit, the plain gather runs instead).
+size_t avg_tile_lds(int ny) {
+  const size_t nl = static_cast<size_t>(kAvgT) + 2 * static_cast<size_t>(ny) + 2;
+  const size_t b = nl * (sizeof(uint64_t) + 5 * sizeof(float));
+  return b <= 64 * 1024 ? b : 0;
+}
+
 // ------------------------------------------------------------------------------------
 // W3QCK3 sweeps (prop_qck3.hpp): the flux of every FLA entry, then the update of the active
 // cells.  The closure copies (:1087-1112, :1240-1247) are reads of the wrapped column; the
@@ -742,7 +839,12 @@ hipError_t xyp3_records(const XyGrid& g, const XyState& s, hipStream_t st) {
 hipError_t xyp3_substep(const XyGrid& g, const XyState& s, int itloc, bool yfirst, bool flcx, bool flcy,
                         hipStream_t st) {
   const dim3 b(256);
-  hipLaunchKernelGGL(avg_kernel, dim3((g.nq + 255) / 256, s.nlist), b, 0, st, g, s, itloc);
+  const size_t lds = avg_tile_lds(g.ny);
+  if (lds > 0)
+    hipLaunchKernelGGL(avg_tiled_kernel, dim3((g.nq + kAvgT - 1) / kAvgT, s.nlist), dim3(kAvgBS),
+                       static_cast<unsigned>(lds), st, g, s, itloc);
+  else
+    hipLaunchKernelGGL(avg_kernel, dim3((g.nq + 255) / 256, s.nlist), b, 0, st, g, s, itloc);
   const Sweep ys{1, 0, 0, g.clsy, g.atrny};
   const Sweep xs{g.ny, g.global, 1, g.clsx, g.atrnx};
   for (int pass = 0; pass < 2; ++pass) {
