@@ -4,7 +4,8 @@
 // Test harness, not part of the library.
 //
 // Usage: devlibm_host_check <fn> [threads] [samples]
-//   one-argument functions (expf logf log10f tanhf sinf cosf, pow025f = powf(x, 0.25)):
+//   one-argument functions (expf logf log10f tanhf sinf cosf, pow025f = powf(x, 0.25), and the
+//   short-chain forms logf_short sinf_short cosf_short pow025_short against the oracle's):
 //   every float input;
 //   two-argument functions (powf atan2f): `samples` random pairs plus structured ones.
 // Prints one line: fn, inputs, dev!=oracle, dev!=cr, dev!=glibc, max ulp vs cr, vs glibc.
@@ -55,6 +56,14 @@ Eval eval1_fast(const std::string& fn, float x) {
   if (fn == "log10f") return {L::log10f(x), wl_log10f(x), 0, 0};
   if (fn == "tanhf") return {L::tanhf(x), wl_tanhf(x), 0, 0};
   if (fn == "pow025f") return {L::powf(x, 0.25f), wl_powf(x, 0.25f), 0, 0};
+  if (fn == "logf_short") return {L::logf_short(x), wl_logf(x), 0, 0};
+  if (fn == "pow025_short") return {L::pow_quarter(x), wl_powf(x, 0.25f), 0, 0};
+  if (fn == "sinf_short" || fn == "cosf_short") {
+    float s, c, so, co;
+    L::sincosf_short(x, &s, &c);
+    wl_sincosf(x, &so, &co);
+    return fn == "sinf_short" ? Eval{s, so, 0, 0} : Eval{c, co, 0, 0};
+  }
   float s, c, so, co;
   L::sincosf(x, &s, &c);
   wl_sincosf(x, &so, &co);
@@ -69,6 +78,14 @@ Eval eval1(const std::string& fn, float x) {
   if (fn == "log10f") return {L::log10f(x), wl_log10f(x), (float)log10l(x), ::log10f(x)};
   if (fn == "tanhf") return {L::tanhf(x), wl_tanhf(x), (float)tanhl(x), ::tanhf(x)};
   if (fn == "pow025f") return {L::powf(x, 0.25f), wl_powf(x, 0.25f), (float)powl(x, 0.25L), ::powf(x, 0.25f)};
+  if (fn == "logf_short") return {L::logf_short(x), wl_logf(x), (float)logl(x), ::logf(x)};
+  if (fn == "pow025_short") return {L::pow_quarter(x), wl_powf(x, 0.25f), (float)powl(x, 0.25L), ::powf(x, 0.25f)};
+  if (fn == "sinf_short" || fn == "cosf_short") {
+    float s, c, so, co;
+    L::sincosf_short(x, &s, &c);
+    wl_sincosf(x, &so, &co);
+    return fn == "sinf_short" ? Eval{s, so, (float)sinl(x), ::sinf(x)} : Eval{c, co, (float)cosl(x), ::cosf(x)};
+  }
   if (fn == "sinf") {
     float s, c, so, co;
     L::sincosf(x, &s, &c);

@@ -2,7 +2,9 @@
 the oracle's restatement (oracle/wwlibm.h), the correctly rounded value (float rounding of
 the host long double function) and the host glibc.
 
-expf must equal glibc on every float input.  For the other functions the library's
+expf must equal glibc on every float input.  The short-chain forms used on the serial chains
+of the source terms (logf_short, sincosf_short, pow_quarter) must equal the standard
+definitions (the oracle's) on every input tested.  For the other functions the library's
 definition must equal the oracle's on every input tested (all 2^32 floats for one-argument
 functions, 4 million pairs for two-argument ones), and stay within 1 ulp of the correctly
 rounded value; how often it differs from glibc is printed (the reference calls glibc, so
@@ -52,7 +54,8 @@ def test_expf_equals_glibc_everywhere(checker):
     assert rc == 0 and v["dev_ne_glibc"] == 0
 
 
-@pytest.mark.parametrize("fn", ["logf", "log10f", "tanhf", "sinf", "cosf", "pow025f"])
+@pytest.mark.parametrize("fn", ["logf", "log10f", "tanhf", "sinf", "cosf", "pow025f", "logf_short",
+                                "sinf_short", "cosf_short", "pow025_short"])
 def test_unary_all_inputs(checker, fn):
     # device vs oracle on every EVERY-th input; vs the references on every 61st of those
     rc, v = run(checker, fn, samples=61, every=EVERY)

@@ -410,5 +410,84 @@ __host__ __device__ inline float cosf(float x) {
   return c;
 }
 
+// ---------------------------------------------------------------------------------------
+// Short-chain forms for the serial chains of the source terms (W3SIN4's stress scan and tail
+// integral, CALC_USTAR), where each call's latency adds to a point's sub-step.  Each returns
+// the same float as the function it stands for on every input: both are correctly rounded on
+// all 2^32 float inputs (tests/test_libm_cpu.py compares them exhaustively with
+// WW3GPU_LIBM_EXHAUSTIVE=1 and on every 16th input by default), so both equal the correctly
+// rounded value.  The evaluations: ln by the same atanh series cut to s^18/19 (truncation
+// below 2^-55) as an fma Horner chain; sin/cos by the same reduction and the Taylor series to
+// r^15 / r^16 (below 2^-54) in fma chains; x^(1/4) for x > 0 as sqrt(sqrt(x)) in double.
+// ---------------------------------------------------------------------------------------
+namespace shortc {
+
+__host__ __device__ inline double log(double x) {
+  const uint64_t u = d2u(x);
+  int k = static_cast<int>((u >> 52) & 0x7ff) - 1023;
+  double m = u2d((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+  if (m > dp::kSqrt2) {
+    m = m * 0.5;
+    k = k + 1;
+  }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double s2 = s * s;
+  double p = dp::coef().inv_odd[8];  // 1/19
+#pragma unroll
+  for (int n = 7; n >= 0; --n) p = __builtin_fma(p, s2, dp::coef().inv_odd[n]);
+  const double s2s = 2.0 * s;
+  const double lm = __builtin_fma(s2s, s2 * p, s2s);
+  const double kd = static_cast<double>(k);
+  return __builtin_fma(kd, dp::kLn2Hi, __builtin_fma(kd, dp::kLn2Lo, lm));
+}
+
+}  // namespace shortc
+
+/// logf(x), the same value on every input.
+__host__ __device__ inline float logf_short(float x) {
+  if (isnan_f(x)) return x + x;
+  if (x < 0.0f) return __builtin_nanf("");
+  if (x == 0.0f) return -__builtin_inff();
+  if (isinf_f(x)) return x;
+  return static_cast<float>(shortc::log(static_cast<double>(x)));
+}
+
+/// powf(x, 0.25f), the same value on every input.
+__host__ __device__ inline float pow_quarter(float x) {
+  if (x > 0.0f && !isinf_f(x)) return static_cast<float>(__builtin_sqrt(__builtin_sqrt(static_cast<double>(x))));
+  return libm::powf(x, 0.25f);
+}
+
+/// sincosf(x), the same values on every input.
+__host__ __device__ inline void sincosf_short(float x, float* s, float* c) {
+  if (isnan_f(x) || isinf_f(x) || (x < 0.0f ? -x : x) >= 0x1p20f) {
+    libm::sincosf(x, s, c);
+    return;
+  }
+  const double xd = static_cast<double>(x);
+  double kd = xd * dp::kInvPio2 + dp::kShift;
+  const uint64_t ki = d2u(kd);
+  kd = kd - dp::kShift;
+  const double r = ((xd - kd * dp::kPio2_1) - kd * dp::kPio2_2) - kd * dp::kPio2_3;
+  const double r2 = r * r;
+  double ps = -dp::inv_fact(15), pc = dp::inv_fact(16);
+#pragma unroll
+  for (int n = 13; n >= 3; n -= 2) ps = __builtin_fma(ps, r2, (n % 4 == 3) ? -dp::inv_fact(n) : dp::inv_fact(n));
+#pragma unroll
+  for (int n = 14; n >= 2; n -= 2) pc = __builtin_fma(pc, r2, (n % 4 == 2) ? -dp::inv_fact(n) : dp::inv_fact(n));
+  const double sr = __builtin_fma(r, r2 * ps, r);
+  const double cr = __builtin_fma(r2, pc, 1.0);
+  double sn, cs;
+  switch (static_cast<int>(ki & 3u)) {
+    case 0: sn = sr; cs = cr; break;
+    case 1: sn = cr; cs = -sr; break;
+    case 2: sn = -sr; cs = -cr; break;
+    default: sn = -cr; cs = sr; break;
+  }
+  *s = static_cast<float>(sn);
+  *c = static_cast<float>(cs);
+}
+
 }  // namespace libm
 }  // namespace ww3gpu

@@ -258,7 +258,7 @@ __device__ __forceinline__ void calc_ustar(const St4Dev& q, float windspeed, flo
     float zz0 = zz00 / libm::powf(omx, 0.50f);
     const float znu = znunum / fmax2(ust, 1E-6f);
     zz0 = q.sintailpar[4] * znu + zz0;
-    const float lg = libm::logf(q.zzwnd / zz0);
+    const float lg = libm::logf_short(q.zzwnd / zz0);
     const float fval = ust - kKappa * windspeed / lg;
     const float delf = 1.f - kKappa * windspeed / (lg * lg) * 2.f / ust * (1.f - xm1 * x) / omx;
     ust = ust - fval / delf;
@@ -517,9 +517,9 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
     for (int ik = 1; ik <= nk; ++ik) {
       const float taupx = taux - __builtin_fabsf(q.ttauwshelter) * ssx;
       const float taupy = tauy - __builtin_fabsf(q.ttauwshelter) * ssy;
-      ustp = mn(libm::powf((taupx * taupx + taupy * taupy), 0.25f), mx(ust, 0.3f));
+      ustp = mn(libm::pow_quarter(taupx * taupx + taupy * taupy), mx(ust, 0.3f));
       usdirp = libm::atan2f(taupy, taupx);
-      libm::sincosf(usdirp, &sinu, &cosu);
+      libm::sincosf_short(usdirp, &sinu, &cosu);
       const int is1 = 1 + (ik - 1) * nth;
       const float s2 = q.sig2[is1 - 1];
       const float cm = pk[ik - 1];
@@ -615,7 +615,7 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
   {
     const float taupx = taux - __builtin_fabsf(q.ttauwshelter) * xstress;
     const float taupy = tauy - __builtin_fabsf(q.ttauwshelter) * ystress;
-    ustp = libm::powf((taupx * taupx + taupy * taupy), 0.25f);
+    ustp = libm::pow_quarter(taupx * taupx + taupy * taupy);
     usdirp = libm::atan2f(taupy, taupx);
   }
   ust = ustp;
@@ -681,16 +681,16 @@ __device__ __forceinline__ void sin4(const Grp& t, const St4Dev& q, Sc& c) {
       const float zx = ustr / cm + q.zzalp;
       const float zarg = mn(kKappa / zx, 20.f);
       const float zmu = mn(tl[2 * TS + j - 1] * libm::expf(zarg), 1.f);
-      const float zlog = mn(libm::logf(zmu), 0.f);
+      const float zlog = mn(libm::logf_short(zmu), 0.f);
       const float zl2 = zlog * zlog;
       const float zbeta = const1 * zmu * (zl2 * zl2);
       float levtail;
       if (sp2 > 0.f) {
         float alpham;
         if (ustr < cm)
-          alpham = mx(0.f, 0.01f * (1.f + libm::logf(ustr / cm)));
+          alpham = mx(0.f, 0.01f * (1.f + libm::logf_short(ustr / cm)));
         else
-          alpham = 0.01f * (1.f + 3.f * libm::logf(ustr / cm));
+          alpham = 0.01f * (1.f + 3.f * libm::logf_short(ustr / cm));
         const float alphat = alpham * tl[3 * TS + j - 1] * tl[4 * TS + j - 1];
         levtail = tl[5 * TS + j - 1] + tl[6 * TS + j - 1] * alphat;
       } else {
