@@ -12,10 +12,12 @@
 // call; validation failures write no output; npts == 0 is a no-op.
 //
 // The sub-step loop.  Every point of a batch integrates to DTG in its own number of
-// sub-steps.  One device sub-step is: k_step_a (W3SLN1, W3SIN4, KDMEAN), the DIA over the
-// points still integrating, k_step_c (W3SDS4, W3SDB1, W3SBT1, the time-step limit, the
-// update, W3SPR4, the tail, W3SIN4) which counts the points that are not done.  The host
-// reads that count (4 bytes) and stops at zero.
+// sub-steps.  k_init runs W3SRCE's section 1 and the first sub-step's input terms (W3SLN1,
+// W3SIN4, KDMEAN); then one device sub-step is the DIA over the points still integrating and
+// k_step (W3SDS4, W3SDB1, W3SBT1, the time-step limit, the update, W3SPR4, the tail, the next
+// sub-step's W3SIN4), which lists and counts the points that are not done.  The host reads
+// that count (4 bytes), sizes the next launches by it and stops at zero; k_finish runs
+// section 3.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
