@@ -4,8 +4,9 @@ Every output of ww_srce_batch (spectrum, the 22 pointer scalars, WHITECAP, TAUBB
 SHAVE) must equal the oracle's bits.  Cases: the 9 520-point NK=32/NTH=36 step at DTG =
 300 s, small cases for each configuration branch (STAB3, IG1, isotropic saturation,
 isotropic cumulative term, depth-limited breaking, CAPCHNK, no LN1, FDONLY, no whitecap
-output, ice, IT = 0), calm and shallow points, the single-point entry and the
-device-resident entry."""
+output, ice, IT = 0), calm and shallow points, the single-point entry, the device-resident
+entry, every group width of the sub-step kernel, and 64 uneven domains in one batch against
+each domain alone."""
 import numpy as np
 import pytest
 
@@ -128,6 +129,31 @@ def test_group_widths(rt, lanes, nk, nth):
     finally:
         L.ww3gpu_srce_group_lanes(0)
     _assert_same(got, ref, f"{lanes} lanes, nk {nk} nth {nth}")
+
+
+def test_uneven_domains_in_one_batch(rt):
+    """The batched-grids configuration at test size: 64 regional domains, each with its own
+    seeded winds, depths and spectra and a log-uniform size (8 to 300 points), integrated as one
+    batch and each alone.  Every domain's outputs are the same bits either way, and the batch
+    equals the oracle."""
+    rng = np.random.default_rng(77)
+    sizes = np.exp(rng.uniform(np.log(8.0), np.log(300.0), 64)).astype(int)
+    cases = [sc.make_case(int(n), nk=32, nth=36, seed=1000 + i) for i, n in enumerate(sizes)]
+    big = dict(cases[0])
+    big["npts"] = int(sizes.sum())
+    for k in ("spec", "wn1", "cg1", "pin", "psc", "whitecap", "taubbl", "tauice", "shave"):
+        big[k] = np.concatenate([c[k] for c in cases])
+    print(f"[srce] {len(cases)} domains, {big['npts']} points in one batch", flush=True)
+    tab = insin4_tables(cases[0])
+    s = Srce(_grid(), big, tab)
+    got = s.batch()
+    _assert_same(got, O.srce_batch(big, tab, nthreads=16), "batch of 64 domains")
+    off = 0
+    for i, c in enumerate(cases):
+        one = s.batch(case=c)
+        n = c["npts"]
+        _assert_same({k: got[k][off:off + n] for k in KEYS}, one, f"domain {i} ({n} points)")
+        off += n
 
 
 def test_calm_and_extreme_points(rt):
