@@ -5,7 +5,8 @@ A VA(NSPEC, 0:NSEALM) is bound per grid; kernels update the device copy and the 
 stale until a sync.  Checked: the HOST/DEVICE/BOTH transitions, W3SRCE on scattered columns
 equal to the host-array entry (and so to the oracle) with the other columns untouched, the
 partial read and write of columns, zeroed columns, the propagation entries on the resident
-copy equal to their host-array forms, and the refusal of a second binding."""
+copy equal to their host-array forms, the refusal of a second binding, and three coupled time
+steps (W3XYP3 then W3SRCE on the resident copy) against the oracle's same sequence."""
 import numpy as np
 import pytest
 
@@ -104,3 +105,45 @@ def test_propagation_va(rt):
     rva, rth, rk = O.ktp3_batch(k, nthreads=16)
     assert np.array_equal(_bits(kva), _bits(rva))
     assert np.array_equal(_bits(cth), _bits(rth)) and np.array_equal(_bits(ck), _bits(rk))
+
+
+def test_coupled_resident_steps(rt):
+    """Three time steps of the resident loop on one grid: W3XYP3 on the resident VA, then W3SRCE
+    on every sea column of it (ww_srce_batch_va), with the spectrum never leaving the device
+    between the calls, bit for bit against the oracle running the same sequence (the
+    propagation plus source-term step at test size)."""
+    L = capi.lib()
+    p = capi.ptr
+    x = pc.make_xyp3(nx=14, ny=12, nk=20, nth=12, seed=71, boundary=True)
+    nspec, nsea = x["nspec"], x["nsea"]
+    c = sc.make_case(nsea, nk=20, nth=12, seed=72)
+    tab = insin4_tables(c)
+    igrid = 911
+    Srce(igrid, c, tab)
+    va = x["va"].copy()
+    ref_va = x["va"].copy()
+    jsea = np.arange(1, nsea + 1, dtype=np.int32)
+    inout = ("psc", "whitecap", "taubbl", "tauice", "shave")
+    outs = {k: c[k].copy() for k in inout}
+    ref = {k: c[k].copy() for k in inout}
+    fl = c["flags"]
+    xargs = prop._args(dict(x, va=va), pc.XYP3_ARGS, prop._XYP3_SCALARS, ("rpar", "ipar", "imaps", "tscal"))
+    for step in range(3):
+        print(f"[resident] step {step}: {nsea} sea points, {nspec} components", flush=True)
+        assert L.ww_xyp3_batch_va(igrid, *xargs[:-2], p(va), nspec, nsea) == capi.Status.OK
+        L.ww_srce_batch_va(igrid, fl["it"], fl["inflags24"], fl["flogrd57"], fl["flogrd58"], fl["dtg"], p(va),
+                           nspec, nsea, nsea, p(jsea), p(c["wn1"]), p(c["cg1"]), p(c["pin"]), p(outs["psc"]),
+                           p(outs["whitecap"]), p(outs["taubbl"]), p(outs["tauice"]), p(outs["shave"]))
+        assert L.ww_srce_last_error() == capi.Status.OK
+        assert L.ww3gpu_va_state(igrid) == DEVICE
+        ref_va = O.xyp3_batch(dict(x, va=ref_va), nthreads=16)
+        rv = ref_va.reshape(nsea + 1, nspec)
+        r = O.srce_batch(dict(c, spec=np.ascontiguousarray(rv[1:]), **ref), tab, nthreads=16)
+        rv[1:] = r["spec"]
+        ref_va = rv.reshape(-1).copy()
+        ref = {k: r[k] for k in inout}
+    assert L.ww_va_host_read(igrid, p(va), nspec, nsea) == capi.Status.OK
+    assert np.array_equal(_bits(va), _bits(ref_va))
+    for k in ("psc", "whitecap", "taubbl", "tauice"):
+        assert np.array_equal(_bits(outs[k]), _bits(ref[k])), k
+    assert np.array_equal(outs["shave"], ref["shave"])
