@@ -40,12 +40,17 @@ struct Rec {
   float rd1, rd2, rd3, rd4;
 };
 
-/// Where a sea point's nine averaging stores go, in the order its loop makes them (centre
-/// first; the central loop's +1..+4, -1..-4 or the near-coast loop's +1, -1, .. -4, each
-/// already sent back to the centre where the reference redirects it): 4 bits per store, the
-/// index of the target's offset in {0, 1, -1, NY, -NY, NY+1, NY-1, -NY+1, -NY-1}; bit 36
-/// marks a near-coast point.
-constexpr int kNearCoastBit = 36;
+/// Where a sea point's nine averaging stores go, grouped by target direction (the index of the
+/// target's offset in {0, 1, -1, NY, -NY, NY+1, NY-1, -NY+1, -NY-1}), each group in the order the
+/// point's loop makes its stores (centre first; the central loop's +1..+4, -1..-4 or the
+/// near-coast loop's +1, -1, .. -4, each already sent back to the centre where the reference
+/// redirects it).  Direction d = 1..8 uses bits 6(d-1) .. 6(d-1)+5: the number of stores there
+/// (0, 1 or 2: the eight stores hold four offsets and their negatives, so a direction gets at
+/// most two) and the factor set - 1 of each (2 bits).  Direction 0 receives the centre store
+/// (factor set 0) and then the near-coast stores j = 1..8 sent back, a mask in bits 48..55
+/// (store j at bit 47 + j).  Bit 62 marks a near-coast point.
+constexpr int kCentreShift = 48;
+constexpr int kNearCoastBit = 62;
 
 /// The grid of a W3XYP3 call, device pointers.  Cell IXY = IY + (IX-1)*NY, 1-based.
 struct XyGrid {

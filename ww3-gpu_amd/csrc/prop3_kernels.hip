@@ -169,7 +169,18 @@ __global__ __launch_bounds__(256) void records_kernel(XyGrid g, XyState s) {
     off[8] = axial(-i4);
   }
   uint64_t w = coast ? (1ULL << kNearCoastBit) : 0ULL;
-  for (int j = 0; j < 9; ++j) w |= static_cast<uint64_t>(dir_index(off[j], ny)) << (4 * j);
+  for (int j = 1; j < 9; ++j) {
+    const int d = dir_index(off[j], ny);
+    const int k = coast ? (j + 1) / 2 : (j - 1) % 4 + 1;
+    if (d == 0) {  // sent back to the centre (near-coast only)
+      w |= 1ULL << (kCentreShift - 1 + j);
+      continue;
+    }
+    const int sh = 6 * (d - 1);
+    const int cnt = static_cast<int>((w >> sh) & 3ULL);
+    w |= static_cast<uint64_t>(k - 1) << (sh + 2 + 2 * cnt);
+    w = (w & ~(3ULL << sh)) | (static_cast<uint64_t>(cnt + 1) << sh);
+  }
   s.tw[o] = w;
   // VLCFLX, VLCFLY at this cell (prop_xyp3.hpp:1080-1137)
   float cxt = (cp.ecosi * cgi) / clat;
@@ -212,24 +223,26 @@ __device__ float avg_fold(const XyGrid& g, const XyState& s, int c, const float*
       continue;
     }
     const uint64_t w = s.tw[rb + src];
-    const bool coast = (w >> kNearCoastBit) & 1ULL;
     const Rec r = s.rec[rb + src];
-#pragma unroll
-    for (int j = 0; j < 9; ++j) {
-      if (static_cast<int>((w >> (4 * j)) & 0xFu) != d) continue;
-      const int k = j == 0 ? 0 : (coast ? (j + 1) / 2 : (j - 1) % 4 + 1);
-      float term;
-      if (k == 0)
-        term = (((3.f - r.rd2) - r.rd4) * a) / 3.f;
-      else if (k == 1)
-        term = ((r.rd2 * a) * r.rd1) / 6.f;
-      else if (k == 2)
-        term = (((1.f - r.rd1) * a) * r.rd2) / 6.f;
-      else if (k == 3)
-        term = ((r.rd4 * a) * r.rd3) / 6.f;
-      else
-        term = (((1.f - r.rd3) * a) * r.rd4) / 6.f;
-      v = v + term;
+    const auto term = [&](int k) {
+      if (k == 0) return (((3.f - r.rd2) - r.rd4) * a) / 3.f;
+      if (k == 1) return ((r.rd2 * a) * r.rd1) / 6.f;
+      if (k == 2) return (((1.f - r.rd1) * a) * r.rd2) / 6.f;
+      if (k == 3) return ((r.rd4 * a) * r.rd3) / 6.f;
+      return (((1.f - r.rd3) * a) * r.rd4) / 6.f;
+    };
+    if (d == 0) {  // the centre store, then the near-coast stores sent back, in store order
+      v = v + term(0);
+      unsigned m = static_cast<unsigned>((w >> kCentreShift) & 0xFFULL);
+      while (m != 0u) {
+        const int j = __builtin_ctz(m) + 1;
+        m &= m - 1u;
+        v = v + term((j + 1) / 2);
+      }
+    } else {
+      const unsigned f = static_cast<unsigned>((w >> (6 * (d - 1))) & 0x3FULL);
+      if ((f & 3u) != 0u) v = v + term(static_cast<int>((f >> 2) & 3u) + 1);
+      if ((f & 3u) == 2u) v = v + term(static_cast<int>((f >> 4) & 3u) + 1);
     }
   }
   return v;
@@ -325,12 +338,18 @@ __global__ __launch_bounds__(kAvgBS) void avg_tiled_kernel(XyGrid g, XyState s, 
         v = tm[i];
         continue;
       }
-      const bool coast = (w >> kNearCoastBit) & 1ULL;
-#pragma unroll
-      for (int q = 0; q < 9; ++q) {
-        if (static_cast<int>((w >> (4 * q)) & 0xFu) != d) continue;
-        const int kk = q == 0 ? 0 : (coast ? (q + 1) / 2 : (q - 1) % 4 + 1);
-        v = v + tm[kk * nl + i];
+      if (d == 0) {  // the centre store, then the near-coast stores sent back, in store order
+        v = v + tm[i];
+        unsigned m = static_cast<unsigned>((w >> kCentreShift) & 0xFFULL);
+        while (m != 0u) {
+          const int j = __builtin_ctz(m) + 1;
+          m &= m - 1u;
+          v = v + tm[((j + 1) / 2) * nl + i];
+        }
+      } else {
+        const unsigned f = static_cast<unsigned>((w >> (6 * (d - 1))) & 0x3FULL);
+        if ((f & 3u) != 0u) v = v + tm[(static_cast<int>((f >> 2) & 3u) + 1) * nl + i];
+        if ((f & 3u) == 2u) v = v + tm[(static_cast<int>((f >> 4) & 3u) + 1) * nl + i];
       }
     }
     if (t >= 1 && t <= g.nxy) {
