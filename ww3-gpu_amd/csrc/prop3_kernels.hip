@@ -295,6 +295,7 @@ __global__ __launch_bounds__(kAvgBS) void avg_tiled_kernel(XyGrid g, XyState s, 
   const float* aq = s.q[(itloc - 1) & 1] + static_cast<size_t>(c) * g.nq;
   float* vq = s.q[itloc & 1] + static_cast<size_t>(c) * g.nq;
   const size_t rb = static_cast<size_t>(c) * (g.nxy + 1);
+#pragma unroll 8
   for (int i = static_cast<int>(threadIdx.x); i < nl; i += kAvgBS) {
     const int src = s0 + i;
     uint64_t w = 0;
