@@ -269,7 +269,8 @@ def bench_srce(args):
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": measured_traffic("srce_step_kernel_hbm_traffic"),
             "kernel": groups[dom]["kernel"], "group": dom, "bytes_per_launch": per_launch,
             "step_bytes_per_point": SRCE_STEP_BYTES,
             "step_achieved_gbs": SRCE_STEP_BYTES * npts / (ms_step * 1e-3) / 1e9,
@@ -385,12 +386,49 @@ def bench_prop(args):
     }
 
 
+def bench_copy(args):
+    """A plain device copy (torch copy_ of 1 GiB): the calibration run for the HBM counters
+    FETCH_SIZE and WRITE_SIZE, whose readings per launch are set against the 1 GiB each way
+    this copy is known to move."""
+    import torch
+
+    n = 1 << 28  # float32: 1 GiB
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    return {
+        "metric": "device copy bandwidth (HBM counter calibration)", "value": 2 * n * 4 / dt / 1e9,
+        "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "torch copy_ of 1 GiB (1 GiB read and 1 GiB written per launch)",
+                   "bytes_read_per_launch": n * 4, "bytes_written_per_launch": n * 4},
+    }
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel as measured offline by the two counter passes and the
+    copy calibration (profiles/<name>.json, written from the rocprofv3 summaries there), or
+    None."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="srce", choices=["srce", "dia", "prop"])
+    ap.add_argument("--workload", default="srce", choices=["srce", "dia", "prop", "copy"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -402,6 +440,10 @@ def main():
         args.steps = 500 if args.steps is None else args.steps
         args.warmup = 50 if args.warmup is None else args.warmup
         res = bench_dia(args)
+    elif args.workload == "copy":
+        args.steps = 5 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        res = bench_copy(args)
     elif args.workload == "prop":
         args.steps = 10 if args.steps is None else args.steps
         args.warmup = 2 if args.warmup is None else args.warmup
