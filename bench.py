@@ -174,7 +174,7 @@ def bench_srce(args):
     from ww3gpu import capi, srce_case as sc
     from ww3gpu.srce import Srce
 
-    nk, nth, npts = 32, 36, 9520
+    nk, nth, npts = 32, 36, args.npts
     c = sc.make_case(npts, nk=nk, nth=nth, seed=2024)
     op = Srce(1, c)
     L = capi.lib()
@@ -261,7 +261,7 @@ def bench_srce(args):
         "data": "synthetic (seeded JONSWAP spectra, Weibull winds, log-uniform depths)",
         "config": {
             "workload": "full ST4 source-term step (W3SRCE: W3SIN4 + W3SDS4 + W3SNL1 + W3SLN1 + "
-                        "W3SBT1 + W3SDB1, semi-implicit sub-steps to DTG): 9520 points, NK=32, "
+                        f"W3SBT1 + W3SDB1, semi-implicit sub-steps to DTG): {npts} points, NK=32, "
                         "NTH=36, DTG=300 s, IT=1",
             "npts": npts, "nk": nk, "nth": nth, "dtg_s": fl["dtg"], "parallelism": "single GPU",
             "substeps_max": substeps, "point_substeps_per_step": pts_substeps,
@@ -270,7 +270,7 @@ def bench_srce(args):
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": measured_traffic("srce_step_kernel_hbm_traffic"),
+            "traffic": measured_traffic("srce_step_kernel_hbm_traffic") if npts == 9520 else None,
             "kernel": groups[dom]["kernel"], "group": dom, "bytes_per_launch": per_launch,
             "step_bytes_per_point": SRCE_STEP_BYTES,
             "step_achieved_gbs": SRCE_STEP_BYTES * npts / (ms_step * 1e-3) / 1e9,
@@ -430,6 +430,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="srce", choices=["srce", "dia", "prop", "copy"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--npts", type=int, default=9520,
+                    help="points of the W3SRCE workload (9520: the metric's configuration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
