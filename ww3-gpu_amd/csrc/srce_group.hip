@@ -40,6 +40,7 @@
 // whitecap sums run after the cumulative term, which does not touch their inputs.  The
 // time-step limit's MIN over bins is formed in lane order: its operands are never NaN or
 // -0 (each is AFAC / MAX(., 1E-10) >= +0), so the order cannot change the result.
+#include <algorithm>
 #include <atomic>
 
 #include "devlibm.hpp"
@@ -1430,10 +1431,10 @@ __global__ __launch_bounds__(64) void init_kernel(State s, St4Dev q, SrceDev pp,
 
 template <int G, int NTH>
 __global__ __launch_bounds__(64) void step_kernel(State s, St4Dev q, SrceDev pp, StepFlags f, int iter,
-                                                  int n) {
+                                                  int first, int n) {
   extern __shared__ float lds[];
   const int g = static_cast<int>(threadIdx.x) / G;
-  const int slot = static_cast<int>(blockIdx.x) * (64 / G) + g;
+  const int slot = first + static_cast<int>(blockIdx.x) * (64 / G) + g;
   if (slot >= n) return;
   const int p = s.list[iter & 1][slot];
   const int nk = pp.nk, nth = pp.nth, nspec = pp.nspec;
@@ -1969,16 +1970,18 @@ hipError_t k_init(const State& s, const St4Dev& q, const SrceDev& p, const StepF
   return hipGetLastError();
 }
 
+/// Step-kernel launch for list slots [first, n).
 template <int G>
-hipError_t launch_step(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, int iter, int n,
-                       hipStream_t st) {
+hipError_t launch_step(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, int iter, int first,
+                       int n, hipStream_t st) {
   const unsigned lds = lds_of<G>(q);
+  const dim3 grid = groups<G>(n - first);
   if (p.nth == 36)
-    hipLaunchKernelGGL((step_kernel<G, 36>), groups<G>(n), dim3(64), lds, st, s, q, p, f, iter, n);
+    hipLaunchKernelGGL((step_kernel<G, 36>), grid, dim3(64), lds, st, s, q, p, f, iter, first, n);
   else if (p.nth == 24)
-    hipLaunchKernelGGL((step_kernel<G, 24>), groups<G>(n), dim3(64), lds, st, s, q, p, f, iter, n);
+    hipLaunchKernelGGL((step_kernel<G, 24>), grid, dim3(64), lds, st, s, q, p, f, iter, first, n);
   else
-    hipLaunchKernelGGL((step_kernel<G, 0>), groups<G>(n), dim3(64), lds, st, s, q, p, f, iter, n);
+    hipLaunchKernelGGL((step_kernel<G, 0>), grid, dim3(64), lds, st, s, q, p, f, iter, first, n);
   return hipGetLastError();
 }
 
@@ -1986,15 +1989,42 @@ namespace {
 std::atomic<int> g_group_lanes{0};  // set_group_lanes
 }  // namespace
 
+/// Points that one round of 16-lane-row workgroups holds on the current device: the CUs times
+/// the workgroups one CU takes (by LDS, at most eight: the step kernel's two wavefronts per
+/// SIMD) times four; 0 if the device cannot be queried.
+int rows_round(const St4Dev& q) {
+  int dev = 0, cus = 0, lds_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+      cus <= 0 || lds_cu <= 0)
+    return 0;
+  const int per_cu = std::min(8, lds_cu / static_cast<int>(lds_of<16>(q)));
+  return cus * per_cu * 4;
+}
+
 hipError_t k_step(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f, int iter, int n,
                   hipStream_t st) {
   // Many points: a 16-lane row per point (four per wavefront) for throughput.  Fewer: two
   // rows per point, then (the late sub-steps) a wavefront per point, which shorten each
-  // point's sub-step while every wavefront still fits the GPU in one round.
+  // point's sub-step while every wavefront still fits the GPU in one round.  Past one round
+  // of rows, the points left over (at most what one round of two-row groups holds) go to a
+  // second launch of two-row groups, which finishes sooner than a second round of rows.
   const int lanes = g_group_lanes.load(std::memory_order_relaxed);
-  if (lanes == 16 || (lanes == 0 && n > kHalfBelow)) return launch_step<16>(s, q, p, f, iter, n, st);
-  if (lanes == 32 || (lanes == 0 && n > kWideBelow)) return launch_step<32>(s, q, p, f, iter, n, st);
-  return launch_step<64>(s, q, p, f, iter, n, st);
+  if (lanes == 16) return launch_step<16>(s, q, p, f, iter, 0, n, st);
+  if (lanes == 32) return launch_step<32>(s, q, p, f, iter, 0, n, st);
+  if (lanes == 64) return launch_step<64>(s, q, p, f, iter, 0, n, st);
+  if (n > kHalfBelow) {
+    const int round16 = rows_round(q);
+    if (round16 > 0 && n > round16 && n - round16 <= kHalfBelow) {
+      const hipError_t e = launch_step<16>(s, q, p, f, iter, 0, round16, st);
+      if (e != hipSuccess) return e;
+      return launch_step<32>(s, q, p, f, iter, round16, n, st);
+    }
+    return launch_step<16>(s, q, p, f, iter, 0, n, st);
+  }
+  if (n > kWideBelow) return launch_step<32>(s, q, p, f, iter, 0, n, st);
+  return launch_step<64>(s, q, p, f, iter, 0, n, st);
 }
 
 void set_group_lanes(int lanes) { g_group_lanes.store(lanes, std::memory_order_relaxed); }
