@@ -1,6 +1,6 @@
 """Benchmark of the spectral-step hot path on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload srce|dia|prop]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload srce|dia|prop|surrogate]
 
 A step is one pass of the hot path over one batch of synthetic sea points, with every
 input already resident in HBM.
@@ -12,6 +12,10 @@ DTG = 300 s, over 9520 points, NK=32, NTH=36, U10 Weibull(k=2, mean 9 m/s) cappe
 same input state (a device-to-device copy of the INOUT arrays, inside the timed region).
 
 Workload "dia": W3SNL1 alone over the same 9520 points.
+
+Workload "surrogate": the learned W3SNL1 surrogate (trained first, on the device, on 4M
+exact-DIA pairs) against the exact DIA on 1M held-out points: throughput, speed-up and the
+S_nl error.
 
 Prints one JSON line.  The CPU baseline is the oracle (a scalar C restatement, OpenMP over
 points) timed on a bounded sample on this host's cores.
@@ -386,6 +390,118 @@ def bench_prop(args):
     }
 
 
+def surrogate_flops_per_point(nspec, hidden=512):
+    # Multiply-adds of the four layers at their algorithmic widths (NSPEC + 1 inputs, NSPEC
+    # outputs), two operations each; the padding to multiples of 128 is not counted.
+    return 2 * ((nspec + 1) * hidden + 2 * hidden * hidden + hidden * nspec)
+
+
+def bench_surrogate(args):
+    """The learned W3SNL1 surrogate against the exact DIA on the same points.
+
+    Trains the network on the device (spectra from the library's seeded generator, half of
+    them bimodal swell + sea; targets from the exact DIA), then times both on --sur-npts
+    held-out points and reports the surrogate's S_nl error against the DIA."""
+    import numpy as np
+    import torch
+
+    from ww3gpu import synth
+    from ww3gpu.snl1 import Snl1
+    from ww3gpu.snlsur import SnlSurrogate
+
+    nk, nth = 32, 36
+    cfg, sig = synth.dia_config(nk, nth)
+    dia = Snl1(5, cfg, sig)
+    sur = SnlSurrogate(dia, seed=1)
+    t0 = time.perf_counter()
+    losses = sur.fit(args.sur_train, 512, args.sur_epochs, 5e-4, seed=5, bimodal=0.5)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    print(f"trained on {args.sur_train} DIA pairs x {args.sur_epochs} passes in {fit_s:.1f} s; "
+          f"last-batch loss per pass {np.array2string(losses, precision=4)}", file=sys.stderr, flush=True)
+
+    def errors(a, cg, kd):
+        s = torch.empty_like(a)
+        d = torch.empty_like(a)
+        dia.device(a, cg, kd, s, d)
+        p, f, nfb = sur.predict(a, cg, kd)
+        torch.cuda.synchronize()
+        rel = (((p - s) ** 2).sum(1).sqrt() / (s ** 2).sum(1).sqrt()).float()
+        q = torch.quantile(rel[:200000], torch.tensor([0.5, 0.9, 0.99], device=rel.device)).tolist()
+        glob = float((((p - s) ** 2).sum() / (s ** 2).sum()).sqrt())
+        return {"rel_l2_global": glob, "rel_l2_point_median": q[0], "rel_l2_point_p90": q[1],
+                "rel_l2_point_p99": q[2], "fallback_fraction": nfb / a.shape[0]}
+
+    npts = args.sur_npts
+    a, cg, kd = sur.synth(npts, seed=4242, bimodal=0.5)
+    err_mixed = errors(a, cg, kd)
+    _, _, an, cgn, kdn = synth.dia_workload(9520, nk, nth, seed=2024, log_depth=(10.0, 4000.0))
+    err_c1 = errors(*(torch.from_numpy(x).cuda() for x in (an, cgn, kdn)))
+
+    stream = torch.cuda.current_stream()
+    s = torch.empty_like(a)
+    d = torch.empty_like(a)
+    for _ in range(args.warmup):
+        dia.device(a, cg, kd, s, d)
+        sur.predict(a, cg, kd)
+    torch.cuda.synchronize()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record(stream)
+    for _ in range(args.steps):
+        dia.device(a, cg, kd, s, d)
+    e1.record(stream)
+    nfb = 0
+    for _ in range(args.steps):
+        _, _, nfb = sur.predict(a, cg, kd)
+    e2.record(stream)
+    torch.cuda.synchronize()
+    dia_ms = e0.elapsed_time(e1) / args.steps
+    sur_ms = e1.elapsed_time(e2) / args.steps
+    fpp = surrogate_flops_per_point(nk * nth)
+    achieved = fpp * npts / (sur_ms * 1e-3) / 1e12
+
+    # CPU baseline: the oracle DIA (what the surrogate replaces) on a bounded sample.
+    import oracle as O
+    t = O.insnl1(nk, nth, cfg["xfr"], cfg["dth"], cfg["lam"], sig)
+    cores = cpu_cores()
+    sample = max(cores * 64, 1024)
+    reps, c0 = 0, time.perf_counter()
+    while True:
+        O.w3snl1(cfg, t, sig, an[:sample], cgn[:sample], kdn[:sample], nthreads=cores)
+        reps += 1
+        if time.perf_counter() - c0 > args.cpu_seconds:
+            break
+    cpu_s = time.perf_counter() - c0
+    return {
+        "metric": METRIC,
+        "value": npts / (sur_ms * 1e-3),
+        "unit": "grid-point S_nl evaluations/s (surrogate, fallback included)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": sur_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (seeded JONSWAP and bimodal swell + sea spectra, log-uniform depths)",
+        "config": {
+            "workload": f"W3SNL1 surrogate MLP (1153 -> 3 x 512 GELU -> 1152, bf16 matrix cores, fp32 "
+                        f"accumulation) with the energy check and exact-DIA fallback, {npts} points, "
+                        f"NK=32, NTH=36; a step is one S_nl evaluation of every point",
+            "npts": npts, "train_pairs": args.sur_train, "train_passes": args.sur_epochs,
+            "train_s": fit_s, "dia_ms_per_step": dia_ms, "speedup_vs_dia": dia_ms / sur_ms,
+            "fallback_points": nfb, "error_vs_dia_mixed_sea": err_mixed,
+            "error_vs_dia_jonswap_9520": err_c1, "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": achieved / 2500.0, "traffic": None, "flops_per_point": fpp,
+            "kernel": "whole surrogate call (four GEMM launches, normalisation, check, fallback)",
+        },
+        "cpu_baseline": {
+            "value": sample * reps / cpu_s, "unit": "grid-point S_nl evaluations/s (exact DIA)",
+            "cores": cores, "kind": "port",
+            "sample": f"{sample} JONSWAP points x {reps} repetitions, {cpu_s:.1f} s, oracle W3SNL1 "
+                      f"(C, -O3, OpenMP over points)",
+        },
+    }
+
+
 def bench_copy(args):
     """A plain device copy (torch copy_ of 1 GiB): the calibration run for the HBM counters
     FETCH_SIZE and WRITE_SIZE, whose readings per launch are set against the 1 GiB each way
@@ -428,10 +544,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="srce", choices=["srce", "dia", "prop", "copy"])
+    ap.add_argument("--workload", default="srce", choices=["srce", "dia", "prop", "copy", "surrogate"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--npts", type=int, default=9520,
                     help="points of the W3SRCE workload (9520: the metric's configuration)")
+    ap.add_argument("--sur-npts", type=int, default=1 << 20, help="surrogate workload: points timed")
+    ap.add_argument("--sur-train", type=int, default=4 << 20, help="surrogate workload: training pairs")
+    ap.add_argument("--sur-epochs", type=int, default=6, help="surrogate workload: training passes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -446,6 +565,10 @@ def main():
         args.steps = 5 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup
         res = bench_copy(args)
+    elif args.workload == "surrogate":
+        args.steps = 10 if args.steps is None else args.steps
+        args.warmup = 2 if args.warmup is None else args.warmup
+        res = bench_surrogate(args)
     elif args.workload == "prop":
         args.steps = 10 if args.steps is None else args.steps
         args.warmup = 2 if args.warmup is None else args.warmup

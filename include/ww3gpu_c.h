@@ -514,6 +514,53 @@ int ww3gpu_prop3_last_error(void);
  * the last W3KTP3 call.  n >= 4. */
 int ww3gpu_prop3_stats(double* out, int n);
 
+/* ---- Learned W3SNL1 surrogate -------------------------------------------------------------
+ * No counterpart in the reference, whose DIA is exact only.  A multilayer perceptron
+ * (NSPEC + 1 inputs padded to a multiple of 128, three GELU layers of 512, NSPEC outputs
+ * padded likewise; bf16 weights and activations with fp32 accumulation on the matrix cores)
+ * maps a point's normalised spectrum and relative depth to S_nl.  It is trained here on the
+ * exact DIA of the same grid; at inference every point whose prediction fails an energy-
+ * conservation check (or whose spectrum cannot be normalised) gets the exact DIA instead.
+ * Each entry needs ww_snl1_init(igrid) first.  Arrays are the DIA's: a, s (npts, NSPEC),
+ * cg (npts, NK), kdmean (npts), point-major. */
+
+/* Random initial network for grid igrid (replaces an existing one). */
+int ww3gpu_snlsur_init(int igrid, unsigned long long seed);
+/* Parameter count; dims4 (optional) gets the padded input width, hidden width, padded output
+ * width and NSPEC.  Negative when igrid has no network. */
+long long ww3gpu_snlsur_nparams(int igrid, int* dims4);
+/* set != 0: load fp32 parameters from host; 0: copy them out.  Order: W1 [512][kin], b1, W2,
+ * b2, W3, b3, W4 [nout][512], b4, each weight row-major [out][in]. */
+int ww3gpu_snlsur_params(int igrid, int set, float* host, long long count);
+/* Training data on the device: seeded JONSWAP sea states (with probability `bimodal` a swell
+ * partition added), a (npts, NSPEC), cg (npts, NK), kdmean (npts). */
+int ww3gpu_snlsur_synth_dev(int igrid, int npts, unsigned long long seed, float bimodal, float* a,
+                            float* cg, float* kdmean, void* stream);
+/* One Adam step on a batch of B points (B a multiple of 128) with DIA targets s; mean_loss
+ * (optional, host) gets the batch's mean relative squared error in the normalised space. */
+int ww3gpu_snlsur_train_dev(int igrid, int B, const float* a, const float* cg, const float* kdmean,
+                            const float* s, float lr, float* mean_loss, void* stream);
+/* Generate nsamples spectra, their exact DIA, and train for `epochs` passes in batches of
+ * `batch` (cosine learning-rate decay from lr to lr/100).  epoch_loss (optional, host,
+ * `epochs` values): mean loss over the last batch of each pass. */
+int ww3gpu_snlsur_fit(int igrid, int nsamples, int batch, int epochs, float lr,
+                      unsigned long long seed, float bimodal, float* epoch_loss, void* stream);
+/* S_nl of npts points on device arrays; flags (optional, device, npts ints) gets 1 where the
+ * exact DIA was used; *nfallback (optional, host) the number of such points.  Synchronises
+ * the stream once per 2^20 points (the fallback count). */
+int ww3gpu_snlsur_dev(int igrid, int npts, const float* a, const float* cg, const float* kdmean,
+                      float* s, int* flags, int* nfallback, void* stream);
+/* Test entries: loss per point and the fp32 parameter gradients for B host feature rows
+ * x (B, kin) and normalised targets y (B, nout); the raw network output o (M, nout) for
+ * M rows x (M, kin); C = A B^T for bf16 A (M, K), B (N, K) on device (M, N multiples of 128,
+ * K of 64). */
+int ww3gpu_snlsur_grads(int igrid, int B, const float* x, const float* y, float* loss, float* grad);
+int ww3gpu_snlsur_forward(int igrid, int M, const float* x, float* o);
+int ww3gpu_sur_gemm_dev(int M, int N, int K, const void* a, int lda, const void* b, int ldb, float* c,
+                        int ldc, void* stream);
+/* The status of the last surrogate call. */
+int ww3gpu_snlsur_last_error(void);
+
 /* Library version string. */
 const char* ww3gpu_version(void);
 

@@ -89,6 +89,12 @@ bool grid_view(int igrid, GridView& v) {
   return true;
 }
 
+/// The configured grid's DIA set-up (SIG, CONS parameters, tables), or nullptr.
+const Tables* tables_of(int igrid) {
+  const Ctx* c = find(igrid);
+  return c == nullptr ? nullptr : &c->t;
+}
+
 }  // namespace snl1
 }  // namespace ww3gpu
 

@@ -103,6 +103,19 @@ SIGNATURES = {
     "ww_ktp3_batch_va": (c_int, [c_int] + _KTP3[:-4] + [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "ww3gpu_prop3_last_error": (c_int, []),
     "ww3gpu_prop3_stats": (c_int, [c_void_p, c_int]),
+    "ww3gpu_snlsur_init": (c_int, [c_int, ctypes.c_ulonglong]),
+    "ww3gpu_snlsur_nparams": (ctypes.c_longlong, [c_int, c_void_p]),
+    "ww3gpu_snlsur_params": (c_int, [c_int, c_int, c_void_p, ctypes.c_longlong]),
+    "ww3gpu_snlsur_synth_dev": (c_int, [c_int, c_int, ctypes.c_ulonglong, c_float] + [c_void_p] * 4),
+    "ww3gpu_snlsur_train_dev": (c_int, [c_int, c_int] + [c_void_p] * 4 + [c_float, c_void_p, c_void_p]),
+    "ww3gpu_snlsur_fit": (c_int, [c_int, c_int, c_int, c_int, c_float, ctypes.c_ulonglong, c_float,
+                                  c_void_p, c_void_p]),
+    "ww3gpu_snlsur_dev": (c_int, [c_int, c_int] + [c_void_p] * 7),
+    "ww3gpu_snlsur_grads": (c_int, [c_int, c_int] + [c_void_p] * 4),
+    "ww3gpu_snlsur_forward": (c_int, [c_int, c_int, c_void_p, c_void_p]),
+    "ww3gpu_sur_gemm_dev": (c_int, [c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int,
+                                    c_void_p]),
+    "ww3gpu_snlsur_last_error": (c_int, []),
 }
 
 _lib = None
