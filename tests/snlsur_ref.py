@@ -1,6 +1,6 @@
 """Float32 reference of the W3SNL1 surrogate network for its tests (torch on the CPU).
 
-The device network rounds its weights, its input features and every hidden activation to
+The activation is GELU in its tanh form.  The device network rounds its weights, its input features and every hidden activation to
 bfloat16 (round to nearest even) and accumulates in fp32; the reference rounds at the same
 places in the forward pass (a straight-through rounding, so autograd sees the identity) and
 runs the backward pass in fp32.  The device also rounds the backward's intermediate
@@ -38,7 +38,7 @@ def forward_backward(layers, x: np.ndarray, y: np.ndarray):
     h = torch.tensor(bf16(x))
     for i, (w, b) in enumerate(params):
         z = h @ w.T + b
-        h = _round(torch.nn.functional.gelu(z)) if i < 3 else z
+        h = _round(torch.nn.functional.gelu(z, approximate="tanh")) if i < 3 else z
     o = h
     yt = torch.tensor(np.asarray(y, np.float32))
     yy = (yt * yt).sum(1)

@@ -35,7 +35,7 @@ inline int round_up(int n, int m) { return (n + m - 1) / m * m; }
 /// Epilogue of the GEMM C[m][n] = sum_k A[m][k] B[n][k] (+ bias[n]).
 enum EpiMode : int {
   kGeluFwd = 0,   ///< z = C + bias -> zf (fp32, optional), gelu(z) -> hb (bf16), its transpose -> ht (optional)
-  kLinearFwd = 1, ///< C + bias -> of (fp32)
+  kLinearFwd = 1, ///< C + bias -> of (fp32), or -> hb (bf16) when of is null
   kPlain = 2,     ///< C -> of (fp32)
   kGeluBwd = 3,   ///< C * gelu'(zf) -> hb (bf16) and its transpose -> ht
 };
@@ -62,16 +62,17 @@ struct Norm {
 };
 
 /// Features (bf16 [Mpad][kin], and its transpose [kin][ldt] when xt is set) and per-point
+/// (NSPEC <= 2048, NK <= 64)
 /// scale/shift/flag from A (npts, NSPEC), CG (npts, NK), KDMEAN (npts).  With s_ref set
 /// also the fp32 targets y [Mpad][nout].  Rows npts..mpad-1 are zero.
 hipError_t prep(const Norm& n, int npts, int mpad, const float* a, const float* cg,
                 const float* kd, const float* s_ref, uint16_t* x, uint16_t* xt, int ldt, float* y,
                 float* scale, int* shift, int* flag, hipStream_t st);
 
-/// S from the network output o [Mpad][nout]; points that were flagged by prep or fail the
+/// S from the network output o (bf16 [Mpad][nout]); points that were flagged by prep or fail the
 /// energy check are appended to list (count in *nlist, which must start at 0) and marked in
 /// flag.
-hipError_t post(const Norm& n, int npts, const float* o, const float* cg, const float* scale,
+hipError_t post(const Norm& n, int npts, const uint16_t* o, const float* cg, const float* scale,
                 const int* shift, int* flag, float* s, int* list, int* nlist, hipStream_t st);
 
 /// Relative squared error per point: loss[p] = |o-y|^2 / |y|^2, and the gradient of

@@ -150,8 +150,13 @@ int forward(Net& n, int mpad, bool train, hipStream_t st) {
   Epi e{};
   e.mode = kLinearFwd;
   e.bias = P(n, n.boff[3]);
-  e.of = n.o.as<float>();
-  e.ldo = n.norm.nout;
+  if (train) {  // the loss reads fp32; inference keeps the output in bf16
+    e.of = n.o.as<float>();
+    e.ldo = n.norm.nout;
+  } else {
+    e.hb = n.o.as<uint16_t>();
+    e.ldh = n.norm.nout;
+  }
   SUR_TRY(gemm_nt(mpad, n.norm.nout, kHidden, in, kHidden, WB(n, 3), kHidden, e, st), "surrogate: output layer");
   return WW_KOKKOS_OK;
 }
@@ -383,7 +388,7 @@ int ww3gpu_snlsur_forward(int igrid, int M, const float* x_host, float* o_host) 
   if (n == nullptr) return fail(WW_KOKKOS_ERR_NOT_INITIALISED, "ww3gpu_snlsur_forward: no surrogate for this grid");
   if (M <= 0 || M % kTile || x_host == nullptr || o_host == nullptr)
     return fail(WW_KOKKOS_ERR_BAD_SHAPE, "ww3gpu_snlsur_forward: M must be a positive multiple of 128");
-  int rc = ensure_work(*n, M, false);
+  int rc = ensure_work(*n, M, true);  // the training form keeps the output in fp32
   if (rc != WW_KOKKOS_OK) return rc;
   const Norm& q = n->norm;
   hipStream_t st = runtime().stream;
@@ -394,7 +399,7 @@ int ww3gpu_snlsur_forward(int igrid, int M, const float* x_host, float* o_host) 
     return fail(WW_KOKKOS_ERR_RUNTIME, "ww3gpu_snlsur_forward: upload");
   }
   rc = to_bf16(M, q.kin, xf.as<float>(), n->x.as<uint16_t>(), nullptr, st) == hipSuccess
-           ? forward(*n, M, false, st)
+           ? forward(*n, M, true, st)
            : fail(WW_KOKKOS_ERR_KERNEL, "ww3gpu_snlsur_forward: features");
   if (rc == WW_KOKKOS_OK &&
       (hipMemcpyAsync(o_host, n->o.p, static_cast<size_t>(M) * q.nout * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -529,7 +534,7 @@ int ww3gpu_snlsur_dev(int igrid, int npts, const float* a, const float* cg, cons
                  n->shift.as<int>(), n->flag.as<int>(), st), "ww3gpu_snlsur_dev: features");
     if ((rc = forward(*n, mpad, false, st)) != WW_KOKKOS_OK) return rc;
     SUR_TRY(hipMemsetAsync(n->nlist.p, 0, sizeof(int), st), "ww3gpu_snlsur_dev: clear");
-    SUR_TRY(post(q, np, n->o.as<float>(), cgc, n->scale.as<float>(), n->shift.as<int>(), n->flag.as<int>(), sc,
+    SUR_TRY(post(q, np, n->o.as<uint16_t>(), cgc, n->scale.as<float>(), n->shift.as<int>(), n->flag.as<int>(), sc,
                  n->list.as<int>(), n->nlist.as<int>(), st), "ww3gpu_snlsur_dev: output");
     if (flags != nullptr)
       SUR_TRY(hipMemcpyAsync(flags + p0, n->flag.p, static_cast<size_t>(np) * 4, hipMemcpyDeviceToDevice, st),

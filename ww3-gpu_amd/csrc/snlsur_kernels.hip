@@ -22,10 +22,17 @@ __device__ __forceinline__ uint16_t to_bf(float f) {
   return static_cast<uint16_t>(u >> 16);
 }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678f)); }
+// GELU in its tanh form, 0.5 z (1 + tanh(c (z + 0.044715 z^3))) with c = sqrt(2/pi),
+// written as z * sigmoid(2 c (z + 0.044715 z^3)): one exponential and one reciprocal.
+__device__ __forceinline__ float gelu(float z) {
+  const float u = 1.5957691f * (z + 0.044715f * z * z * z);
+  return z * __frcp_rn(1.0f + __expf(-u));
+}
 
 __device__ __forceinline__ float gelu_d(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678f)) + z * 0.39894228f * expf(-0.5f * z * z);
+  const float u = 1.5957691f * (z + 0.044715f * z * z * z);
+  const float sg = __frcp_rn(1.0f + __expf(-u));
+  return sg + z * sg * (1.0f - sg) * 1.5957691f * (1.0f + 0.134145f * z * z);
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -46,14 +53,27 @@ __device__ __forceinline__ float wave_sum(float v) {
 constexpr int kRowB = kTileK * 2 + 16;
 constexpr int kStageB = 2 * kTile * kRowB;
 
+// Persistent tiles: a grid of G workgroups (two per CU) walks the T output tiles, tile
+// index t = blockIdx.x, + G, + 2G, ...  While a tile's epilogue runs, the first A and B
+// slabs of the next tile are already in flight.
+// Tile order: workgroups are dealt to the eight XCDs in turn (workgroup g to XCD g % 8), and
+// each XCD has its own L2.  Tile (t % 8) * (T / 8) + t / 8 (n fastest within an m row)
+// gives every XCD a contiguous run of tiles (G is a multiple of 8), so the N / 128 tiles
+// that read the same rows of A run on one XCD, close together in time, and share them
+// through its L2.
+__device__ __forceinline__ int tile_of(int t, int T) { return T % 8 == 0 ? (t % 8) * (T / 8) + t / 8 : t; }
+
 template <int MODE>
-__global__ __launch_bounds__(256) void gemm_nt_kernel(int K, const uint16_t* __restrict__ A, int lda,
+__global__ __launch_bounds__(256) void gemm_nt_kernel(int K, int ntn, int T, const uint16_t* __restrict__ A, int lda,
                                                       const uint16_t* __restrict__ B, int ldb, Epi e) {
   __shared__ __attribute__((aligned(16))) unsigned char lds[2 * kStageB];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w & 1, wn = w >> 1;
-  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  const int G = static_cast<int>(gridDim.x);
+  int t = static_cast<int>(blockIdx.x);
+  if (t >= T) return;
   const int lr = tid >> 3, lc = tid & 7;
+  int m0 = (tile_of(t, T) / ntn) * kTile, n0 = (tile_of(t, T) % ntn) * kTile;
   const uint16_t* ga = A + static_cast<size_t>(m0 + lr) * lda + lc * 8;
   const uint16_t* gb = B + static_cast<size_t>(n0 + lr) * ldb + lc * 8;
 
@@ -74,84 +94,117 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int K, const uint16_t* __r
     SUR_ST(0, sa_, sb_) SUR_ST(1, sa_, sb_) SUR_ST(2, sa_, sb_) SUR_ST(3, sa_, sb_)           \
   }
 
-  f32x4 acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-
   const int nkt = K / kTileK;
   SUR_GLOAD(0)
   SUR_SSTORE(0)
   __syncthreads();
-  for (int kt = 0; kt < nkt; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nkt) {
-      SUR_GLOAD((kt + 1) * kTileK)
-    }
-    const unsigned char* sa = lds + cur * kStageB;
-    const unsigned char* sb = sa + kTile * kRowB;
+  while (true) {
+    f32x4 acc[4][4];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      bf16x8 fa[4], fb[4];
-      const int koff = s * 64 + (lane >> 4) * 16;
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        fa[i] = *reinterpret_cast<const bf16x8*>(sa + (wm * 64 + i * 16 + (lane & 15)) * kRowB + koff);
+      for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int kt = 0; kt < nkt; ++kt) {
+      const int cur = kt & 1;
+      if (kt + 1 < nkt) {
+        SUR_GLOAD((kt + 1) * kTileK)
+      }
+      const unsigned char* sa = lds + cur * kStageB;
+      const unsigned char* sb = sa + kTile * kRowB;
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        fb[j] = *reinterpret_cast<const bf16x8*>(sb + (wn * 64 + j * 16 + (lane & 15)) * kRowB + koff);
+      for (int s = 0; s < 2; ++s) {
+        bf16x8 fa[4], fb[4];
+        const int koff = s * 64 + (lane >> 4) * 16;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i)
+          fa[i] = *reinterpret_cast<const bf16x8*>(sa + (wm * 64 + i * 16 + (lane & 15)) * kRowB + koff);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+          fb[j] = *reinterpret_cast<const bf16x8*>(sb + (wn * 64 + j * 16 + (lane & 15)) * kRowB + koff);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+      }
+      if (kt + 1 < nkt) SUR_SSTORE(cur ^ 1)
+      __syncthreads();
     }
-    if (kt + 1 < nkt) SUR_SSTORE(cur ^ 1)
-    __syncthreads();
-  }
+
+    // Next tile's first slab into registers before this tile's epilogue.
+    const int tn = t + G;
+    const bool more = tn < T;
+    const int em0 = m0, en0 = n0;
+    if (more) {
+      m0 = (tile_of(tn, T) / ntn) * kTile;
+      n0 = (tile_of(tn, T) % ntn) * kTile;
+      ga = A + static_cast<size_t>(m0 + lr) * lda + lc * 8;
+      gb = B + static_cast<size_t>(n0 + lr) * ldb + lc * 8;
+      SUR_GLOAD(0)
+    }
 
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 4; ++i) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int n = n0 + wn * 64 + j * 16 + (lane & 15);
-      const int mb = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
-      float bn = 0.f;
-      if constexpr (MODE == kGeluFwd || MODE == kLinearFwd) bn = e.bias[n];
-      uint16_t hv[4];
+      for (int j = 0; j < 4; ++j) {
+        const int n = en0 + wn * 64 + j * 16 + (lane & 15);
+        const int mb = em0 + wm * 64 + i * 16 + 4 * (lane >> 4);
+        float bn = 0.f;
+        if constexpr (MODE == kGeluFwd || MODE == kLinearFwd) bn = e.bias[n];
+        uint16_t hv[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const float c = acc[i][j][r];
-        const size_t m = static_cast<size_t>(mb + r);
-        if constexpr (MODE == kGeluFwd) {
-          const float z = c + bn;
-          if (e.zf != nullptr) e.zf[m * e.ldz + n] = z;
-          hv[r] = to_bf(gelu(z));
-          e.hb[m * e.ldh + n] = hv[r];
-        } else if constexpr (MODE == kLinearFwd) {
-          e.of[m * e.ldo + n] = c + bn;
-        } else if constexpr (MODE == kPlain) {
-          e.of[m * e.ldo + n] = c;
-        } else {
-          hv[r] = to_bf(c * gelu_d(e.zf[m * e.ldz + n]));
-          e.hb[m * e.ldh + n] = hv[r];
+        for (int r = 0; r < 4; ++r) {
+          const float c = acc[i][j][r];
+          const size_t m = static_cast<size_t>(mb + r);
+          if constexpr (MODE == kGeluFwd) {
+            const float z = c + bn;
+            if (e.zf != nullptr) e.zf[m * e.ldz + n] = z;
+            hv[r] = to_bf(gelu(z));
+            e.hb[m * e.ldh + n] = hv[r];
+          } else if constexpr (MODE == kLinearFwd) {
+            if (e.of != nullptr)
+              e.of[m * e.ldo + n] = c + bn;
+            else
+              e.hb[m * e.ldh + n] = to_bf(c + bn);
+          } else if constexpr (MODE == kPlain) {
+            e.of[m * e.ldo + n] = c;
+          } else {
+            hv[r] = to_bf(c * gelu_d(e.zf[m * e.ldz + n]));
+            e.hb[m * e.ldh + n] = hv[r];
+          }
         }
-      }
-      if constexpr (MODE == kGeluFwd || MODE == kGeluBwd) {
-        if (e.ht != nullptr) {
-          uint2 pk;
-          pk.x = static_cast<uint32_t>(hv[0]) | (static_cast<uint32_t>(hv[1]) << 16);
-          pk.y = static_cast<uint32_t>(hv[2]) | (static_cast<uint32_t>(hv[3]) << 16);
-          *reinterpret_cast<uint2*>(e.ht + static_cast<size_t>(n) * e.ldt + mb) = pk;
+        if constexpr (MODE == kGeluFwd || MODE == kGeluBwd) {
+          if (e.ht != nullptr) {
+            uint2 pk;
+            pk.x = static_cast<uint32_t>(hv[0]) | (static_cast<uint32_t>(hv[1]) << 16);
+            pk.y = static_cast<uint32_t>(hv[2]) | (static_cast<uint32_t>(hv[3]) << 16);
+            *reinterpret_cast<uint2*>(e.ht + static_cast<size_t>(n) * e.ldt + mb) = pk;
+          }
         }
       }
     }
+    if (!more) break;
+    // Every wave left the k loop through its final barrier, so no wave still reads LDS.
+    SUR_SSTORE(0)
+    __syncthreads();
+    t = tn;
   }
+#undef SUR_LD
+#undef SUR_GLOAD
+#undef SUR_ST
+#undef SUR_SSTORE
 }
 
 // ---------------------------------------------------------------------------------------
 // Normalisation around the network: one wavefront per point, four points per workgroup.
+// A point's spectrum is read from HBM once, as UE = A / CONX, into LDS; the rotated
+// features, the scale and (training) the targets are formed from there.  The divisions by
+// CONX are multiplications by one reciprocal per frequency: the surrogate's inputs need no
+// bit parity with W3SNL1.
+constexpr int kMaxSpec = 2048;  // NSPEC limit of the normalisation kernels
+constexpr int kMaxK = 64;       // NK limit
+
 __device__ __forceinline__ float conx_of(const Norm& nm, const float* cgr, int k) {
   return nm.tpiinv / nm.sig[k] * cgr[k];
 }
@@ -161,38 +214,35 @@ __global__ __launch_bounds__(256) void prep_kernel(Norm nm, int npts, const floa
                                                    const float* __restrict__ s_ref, uint16_t* __restrict__ x,
                                                    uint16_t* __restrict__ xt, int ldt, float* __restrict__ y,
                                                    float* scale, int* shift, int* flag) {
-  const int lane = threadIdx.x & 63;
-  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ float row[4][kMaxSpec];
+  __shared__ float icx[4][kMaxK];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * 4 + wv;
+  const bool live = p < npts;
   const int nspec = nm.nspec, nth = nm.nth, kin = nm.kin, nout = nm.nout;
   uint16_t* xr = x + static_cast<size_t>(p) * kin;
   float* yr = y != nullptr ? y + static_cast<size_t>(p) * nout : nullptr;
-  if (p >= npts) {
-    for (int i = lane; i < kin; i += 64) {
-      xr[i] = 0;
-      if (xt != nullptr) xt[static_cast<size_t>(i) * ldt + p] = 0;
-    }
-    if (yr != nullptr)
-      for (int i = lane; i < nout; i += 64) yr[i] = 0.f;
-    if (lane == 0) {
-      scale[p] = 0.f;
-      shift[p] = 0;
-      flag[p] = 1;
-    }
-    return;
-  }
   const float* ar = a + static_cast<size_t>(p) * nspec;
   const float* cr = cg + static_cast<size_t>(p) * nm.nk;
+  float* ue = row[wv];
+  if (live && lane < nm.nk) icx[wv][lane] = 1.0f / conx_of(nm, cr, lane);
+  __syncthreads();
   float best = -1.0f;
   int bi = 0x7fffffff;
-  bool bad = false;
-  for (int i = lane; i < nspec; i += 64) {
-    const float ue = ar[i] / conx_of(nm, cr, i / nth);
-    if (!(ue >= 0.0f) || isinf(ue)) bad = true;
-    if (ue > best) {
-      best = ue;
-      bi = i;
+  bool bad = !live;
+  if (live) {
+#pragma unroll 6
+    for (int i = lane; i < nspec; i += 64) {
+      const float u = ar[i] * icx[wv][i / nth];
+      ue[i] = u;
+      if (!(u >= 0.0f) || isinf(u)) bad = true;
+      if (u > best) {
+        best = u;
+        bi = i;
+      }
     }
   }
+  __syncthreads();
   for (int off = 32; off >= 1; off >>= 1) {
     const float ob = __shfl_xor(best, off);
     const int oi = __shfl_xor(bi, off);
@@ -201,21 +251,29 @@ __global__ __launch_bounds__(256) void prep_kernel(Norm nm, int npts, const floa
       bi = oi;
     }
   }
-  const float kdp = kd[p];
+  const float kdp = live ? kd[p] : 1.0f;
   bad = __any(bad) || !(best > 0.0f) || !(kdp > 0.0f) || isinf(kdp);
-  const float m = best;
   const int jp = bad ? 0 : bi % nth;
-  const float inv_m = bad ? 0.0f : 1.0f / m;
-  // q = sum over bins of f_k^11 u^3, summed in the rotated order so that a rotated input
-  // gives the same q bit for bit.
+  const float inv_m = bad ? 0.0f : 1.0f / best;
+  // Features in the rotated order, and q = sum f_k^11 u^3 in that order, so that a rotated
+  // input gives the same q bit for bit.
   float q = 0.0f;
-  for (int i = lane; i < nspec && !bad; i += 64) {
+#pragma unroll 6
+  for (int i = lane; i < nspec; i += 64) {
     const int k = i / nth, t = i - k * nth;
     int ts = t + jp;
     if (ts >= nth) ts -= nth;
-    const float u = ar[k * nth + ts] / conx_of(nm, cr, k) * inv_m;
+    const float u = bad ? 0.0f : ue[k * nth + ts] * inv_m;
     const float f = nm.sig[k] * nm.tpiinv, f2 = f * f, f4 = f2 * f2, f8 = f4 * f4;
     q += f8 * f2 * f * (u * u * u);
+    const uint16_t h = to_bf(u);
+    xr[i] = h;
+    if (xt != nullptr) xt[static_cast<size_t>(i) * ldt + p] = h;
+  }
+  for (int i = nspec + lane; i < kin; i += 64) {
+    const uint16_t h = to_bf(i == nspec && !bad ? 0.25f * logf(kdp) : 0.0f);
+    xr[i] = h;
+    if (xt != nullptr) xt[static_cast<size_t>(i) * ldt + p] = h;
   }
   q = wave_sum(q);
   float sc = 0.0f;
@@ -223,56 +281,54 @@ __global__ __launch_bounds__(256) void prep_kernel(Norm nm, int npts, const floa
     const float xx = fmaxf(nm.kdcon * kdp, nm.kdmn);
     const float cons =
         nm.snlc1 * (1.0f + nm.snls1 / xx * (1.0f - nm.snls2 * xx) * expf(fmaxf(-1.e15f, nm.snls3 * xx)));
-    sc = cons * (m * m * m) * (kScaleC0 * q);
+    sc = cons * (best * best * best) * (kScaleC0 * q);
     if (!(sc > 0.0f) || isinf(sc)) bad = true;
   }
-  const float inv_sc = bad ? 0.0f : 1.0f / sc;
-  for (int i = lane; i < nspec; i += 64) {
-    const int k = i / nth, t = i - k * nth;
-    int ts = t + jp;
-    if (ts >= nth) ts -= nth;
-    const int src = k * nth + ts;
-    const float conx = conx_of(nm, cr, k);
-    const float feat = bad ? 0.0f : ar[src] / conx * inv_m;
-    const uint16_t h = to_bf(feat);
-    xr[i] = h;
-    if (xt != nullptr) xt[static_cast<size_t>(i) * ldt + p] = h;
-    if (yr != nullptr) yr[i] = bad ? 0.0f : s_ref[static_cast<size_t>(p) * nspec + src] / conx * inv_sc;
+  if (yr != nullptr) {
+    const float inv_sc = bad ? 0.0f : 1.0f / sc;
+    for (int i = lane; i < nout; i += 64) {
+      float v = 0.0f;
+      if (i < nspec && !bad) {
+        const int k = i / nth, t = i - k * nth;
+        int ts = t + jp;
+        if (ts >= nth) ts -= nth;
+        v = s_ref[static_cast<size_t>(p) * nspec + k * nth + ts] * icx[wv][k] * inv_sc;
+      }
+      yr[i] = v;
+    }
   }
-  for (int i = nspec + lane; i < kin; i += 64) {
-    const uint16_t h = to_bf(i == nspec && !bad ? 0.25f * logf(kdp) : 0.0f);
-    xr[i] = h;
-    if (xt != nullptr) xt[static_cast<size_t>(i) * ldt + p] = h;
-  }
-  if (yr != nullptr)
-    for (int i = nspec + lane; i < nout; i += 64) yr[i] = 0.0f;
   if (lane == 0) {
-    scale[p] = sc;
+    scale[p] = bad ? 0.0f : sc;
     shift[p] = jp;
     flag[p] = bad ? 1 : 0;
   }
 }
 
-__global__ __launch_bounds__(256) void post_kernel(Norm nm, int npts, const float* __restrict__ o,
+__global__ __launch_bounds__(256) void post_kernel(Norm nm, int npts, const uint16_t* __restrict__ o,
                                                    const float* __restrict__ cg, const float* __restrict__ scale,
                                                    const int* __restrict__ shift, int* flag, float* __restrict__ s,
                                                    int* list, int* nlist) {
-  const int lane = threadIdx.x & 63;
-  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (p >= npts) return;
-  const int nspec = nm.nspec, nth = nm.nth;
-  const float sc = scale[p];
-  const int jp = shift[p];
-  const float* orow = o + static_cast<size_t>(p) * nm.nout;
+  __shared__ float cx[4][kMaxK];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int p = blockIdx.x * 4 + wv;
+  const bool live = p < npts;
   const float* cr = cg + static_cast<size_t>(p) * nm.nk;
+  const float sc = live ? scale[p] : 0.0f;
+  if (live && lane < nm.nk) cx[wv][lane] = conx_of(nm, cr, lane) * sc;
+  __syncthreads();
+  if (!live) return;
+  const int nspec = nm.nspec, nth = nm.nth;
+  const int jp = shift[p];
+  const uint16_t* orow = o + static_cast<size_t>(p) * nm.nout;
   float* sr = s + static_cast<size_t>(p) * nspec;
   float er = 0.0f, ea = 0.0f;
   bool nonfinite = false;
+#pragma unroll 6
   for (int i = lane; i < nspec; i += 64) {
     const int k = i / nth, t = i - k * nth;
     int tc = t - jp;
     if (tc < 0) tc += nth;
-    const float v = orow[k * nth + tc] * sc * conx_of(nm, cr, k);
+    const float v = __uint_as_float(static_cast<uint32_t>(orow[k * nth + tc]) << 16) * cx[wv][k];
     sr[i] = v;
     const float wk = nm.wsig[k];
     er += v * wk;
@@ -476,12 +532,23 @@ __global__ __launch_bounds__(256) void synth_kernel(int nk, int nth, const float
 hipError_t gemm_nt(int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B, int ldb,
                    const Epi& e, hipStream_t st) {
   if (M <= 0 || N <= 0 || K <= 0 || M % kTile || N % kTile || K % kTileK) return hipErrorInvalidValue;
-  const dim3 grid(N / kTile, M / kTile), block(256);
+  const int ntn = N / kTile;
+  const long tiles = static_cast<long>(M / kTile) * ntn;
+  if (tiles > 0x7fffffffL) return hipErrorInvalidValue;
+  static const int cus = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      n = 256;
+    return n;
+  }();
+  const int T = static_cast<int>(tiles);
+  const int G = T < 2 * cus ? T : (2 * cus) / 8 * 8;  // two workgroups per CU, a multiple of 8
+  const dim3 grid(static_cast<unsigned>(G)), block(256);
   switch (e.mode) {
-    case kGeluFwd: gemm_nt_kernel<kGeluFwd><<<grid, block, 0, st>>>(K, A, lda, B, ldb, e); break;
-    case kLinearFwd: gemm_nt_kernel<kLinearFwd><<<grid, block, 0, st>>>(K, A, lda, B, ldb, e); break;
-    case kPlain: gemm_nt_kernel<kPlain><<<grid, block, 0, st>>>(K, A, lda, B, ldb, e); break;
-    case kGeluBwd: gemm_nt_kernel<kGeluBwd><<<grid, block, 0, st>>>(K, A, lda, B, ldb, e); break;
+    case kGeluFwd: gemm_nt_kernel<kGeluFwd><<<grid, block, 0, st>>>(K, ntn, T, A, lda, B, ldb, e); break;
+    case kLinearFwd: gemm_nt_kernel<kLinearFwd><<<grid, block, 0, st>>>(K, ntn, T, A, lda, B, ldb, e); break;
+    case kPlain: gemm_nt_kernel<kPlain><<<grid, block, 0, st>>>(K, ntn, T, A, lda, B, ldb, e); break;
+    case kGeluBwd: gemm_nt_kernel<kGeluBwd><<<grid, block, 0, st>>>(K, ntn, T, A, lda, B, ldb, e); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -490,13 +557,14 @@ hipError_t gemm_nt(int M, int N, int K, const uint16_t* A, int lda, const uint16
 hipError_t prep(const Norm& n, int npts, int mpad, const float* a, const float* cg, const float* kd,
                 const float* s_ref, uint16_t* x, uint16_t* xt, int ldt, float* y, float* scale, int* shift,
                 int* flag, hipStream_t st) {
-  if (mpad % 4 || npts > mpad) return hipErrorInvalidValue;
+  if (mpad % 4 || npts > mpad || n.nspec > kMaxSpec || n.nk > kMaxK) return hipErrorInvalidValue;
   prep_kernel<<<mpad / 4, 256, 0, st>>>(n, npts, a, cg, kd, s_ref, x, xt, ldt, y, scale, shift, flag);
   return hipGetLastError();
 }
 
-hipError_t post(const Norm& n, int npts, const float* o, const float* cg, const float* scale,
+hipError_t post(const Norm& n, int npts, const uint16_t* o, const float* cg, const float* scale,
                 const int* shift, int* flag, float* s, int* list, int* nlist, hipStream_t st) {
+  if (n.nspec > kMaxSpec || n.nk > kMaxK) return hipErrorInvalidValue;
   post_kernel<<<(npts + 3) / 4, 256, 0, st>>>(n, npts, o, cg, scale, shift, flag, s, list, nlist);
   return hipGetLastError();
 }
