@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void prep_kernel(Norm nm, int npts, const floa
                                                    const float* __restrict__ s_ref, uint16_t* __restrict__ x,
                                                    uint16_t* __restrict__ xt, int ldt, float* __restrict__ y,
                                                    float* scale, int* shift, int* flag) {
-  __shared__ float row[4][kMaxSpec];
+  extern __shared__ float rows[];  // 4 x NSPEC floats: each wavefront's UE (sized at launch)
   __shared__ float icx[4][kMaxK];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int p = blockIdx.x * 4 + wv;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void prep_kernel(Norm nm, int npts, const floa
   float* yr = y != nullptr ? y + static_cast<size_t>(p) * nout : nullptr;
   const float* ar = a + static_cast<size_t>(p) * nspec;
   const float* cr = cg + static_cast<size_t>(p) * nm.nk;
-  float* ue = row[wv];
+  float* ue = rows + wv * nspec;
   if (live && lane < nm.nk) icx[wv][lane] = 1.0f / conx_of(nm, cr, lane);
   __syncthreads();
   float best = -1.0f;
@@ -558,7 +558,8 @@ hipError_t prep(const Norm& n, int npts, int mpad, const float* a, const float* 
                 const float* s_ref, uint16_t* x, uint16_t* xt, int ldt, float* y, float* scale, int* shift,
                 int* flag, hipStream_t st) {
   if (mpad % 4 || npts > mpad || n.nspec > kMaxSpec || n.nk > kMaxK) return hipErrorInvalidValue;
-  prep_kernel<<<mpad / 4, 256, 0, st>>>(n, npts, a, cg, kd, s_ref, x, xt, ldt, y, scale, shift, flag);
+  prep_kernel<<<mpad / 4, 256, 4 * n.nspec * sizeof(float), st>>>(n, npts, a, cg, kd, s_ref, x, xt, ldt, y, scale,
+                                                                   shift, flag);
   return hipGetLastError();
 }
 
