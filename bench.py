@@ -13,8 +13,8 @@ same input state (a device-to-device copy of the INOUT arrays, inside the timed 
 
 Workload "dia": W3SNL1 alone over the same 9520 points.
 
-Workload "surrogate": the learned W3SNL1 surrogate (trained first, on the device, on 4M
-exact-DIA pairs) against the exact DIA on 1M held-out points: throughput, speed-up and the
+Workload "surrogate": the learned W3SNL1 surrogate (trained first, on the device, on 8M
+exact-DIA pairs, 12 passes) against the exact DIA on 1M held-out points: throughput, speed-up and the
 S_nl error.
 
 Prints one JSON line.  The CPU baseline is the oracle (a scalar C restatement, OpenMP over
@@ -549,8 +549,8 @@ def main():
     ap.add_argument("--npts", type=int, default=9520,
                     help="points of the W3SRCE workload (9520: the metric's configuration)")
     ap.add_argument("--sur-npts", type=int, default=1 << 20, help="surrogate workload: points timed")
-    ap.add_argument("--sur-train", type=int, default=4 << 20, help="surrogate workload: training pairs")
-    ap.add_argument("--sur-epochs", type=int, default=6, help="surrogate workload: training passes")
+    ap.add_argument("--sur-train", type=int, default=8 << 20, help="surrogate workload: training pairs")
+    ap.add_argument("--sur-epochs", type=int, default=12, help="surrogate workload: training passes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
