@@ -187,6 +187,11 @@ def bench_srce(args):
     src = {k: torch.from_numpy(np.ascontiguousarray(c[k])).cuda() for k in keys}
     work = {k: v.clone() for k, v in src.items()}
     inout = ("spec", "psc", "whitecap", "taubbl", "tauice", "shave")
+    tuned = None
+    if args.autotune:
+        from ww3gpu import autotune
+        r = autotune.tune_srce_widths(op, work, src, fl)
+        tuned = {"best": list(r["best"]), "ms_per_step": {f"{h},{w}": v for (h, w), v in r["ms"].items()}}
     stream = torch.cuda.Stream()
     torch.cuda.synchronize()
 
@@ -270,6 +275,7 @@ def bench_srce(args):
             "npts": npts, "nk": nk, "nth": nth, "dtg_s": fl["dtg"], "parallelism": "single GPU",
             "substeps_max": substeps, "point_substeps_per_step": pts_substeps,
             "wall_s": wall, "kernel_groups": groups,
+            "width_thresholds_autotuned": tuned,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -548,6 +554,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--npts", type=int, default=9520,
                     help="points of the W3SRCE workload (9520: the metric's configuration)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="srce workload: measure the sub-step width thresholds first and use the fastest")
     ap.add_argument("--sur-npts", type=int, default=1 << 20, help="surrogate workload: points timed")
     ap.add_argument("--sur-train", type=int, default=8 << 20, help="surrogate workload: training pairs")
     ap.add_argument("--sur-epochs", type=int, default=12, help="surrogate workload: training passes")

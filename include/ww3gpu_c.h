@@ -273,6 +273,13 @@ long ww3gpu_srce_point_substeps(int igrid);
  * OK or BAD_SHAPE. */
 int ww3gpu_srce_group_lanes(int lanes);
 
+/* Tuning control, with no counterpart in the reference: the active-point counts at or below
+ * which the automatic width choice takes two rows per point (half_below) and a wavefront per
+ * point (wide_below), for every later call of every grid; 0, 0 restores the defaults (4096,
+ * 2048).  ww3gpu/autotune.py measures candidates and sets the fastest.  Results do not depend
+ * on it.  Returns BAD_SHAPE for negative values or wide_below > half_below. */
+int ww3gpu_srce_width_thresholds(int half_below, int wide_below);
+
 /* Per-kernel timing of grid igrid's W3SRCE calls.  enable = 1 starts it (and clears the
  * timers), 0 stops it.  While on, every launch of a call is bracketed by HIP events on the
  * call's stream.  Returns OK or NOT_INITIALISED. */

@@ -204,3 +204,13 @@ def test_group_lanes_control():
     for bad in (-1, 8, 48, 128):
         assert L.ww3gpu_srce_group_lanes(bad) == capi.Status.BAD_SHAPE
     assert L.ww3gpu_srce_group_lanes(0) == capi.Status.OK
+
+
+def test_width_thresholds_control():
+    """ww3gpu_srce_width_thresholds: 0, 0 restores the defaults; negative values or
+    wide_below > half_below are BAD_SHAPE (host-only; no device needed)."""
+    L = capi.lib()
+    assert L.ww3gpu_srce_width_thresholds(-1, 0) == capi.Status.BAD_SHAPE
+    assert L.ww3gpu_srce_width_thresholds(100, 200) == capi.Status.BAD_SHAPE
+    assert L.ww3gpu_srce_width_thresholds(5000, 2500) == capi.Status.OK
+    assert L.ww3gpu_srce_width_thresholds(0, 0) == capi.Status.OK

@@ -25,10 +25,10 @@
 namespace ww3gpu {
 namespace srce {
 
-/// Active-point counts at or below which a sub-step runs two 16-lane rows per point instead
-/// of one (kHalfBelow), and a wavefront per point (kWideBelow).  At two wavefronts per SIMD
-/// (the step kernel's register use) the GPU holds 4096 two-row points or 2048 wavefront
-/// points at once.
+/// Default active-point counts at or below which a sub-step runs two 16-lane rows per point
+/// instead of one (kHalfBelow), and a wavefront per point (kWideBelow).  At two wavefronts
+/// per SIMD (the step kernel's register use) the GPU holds 4096 two-row points or 2048
+/// wavefront points at once.  set_width_thresholds replaces them (the autotuner's choice).
 constexpr int kHalfBelow = 4096;
 constexpr int kWideBelow = 2048;
 
@@ -145,8 +145,11 @@ hipError_t k_init(const State& s, const St4Dev& q, const SrceDev& p, const StepF
 hipError_t k_step(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f,
                   int iter, int n, hipStream_t st);
 /// The group width of k_step: 0 chooses it per sub-step from the active-point count
-/// (kHalfBelow, kWideBelow); 16, 32 or 64 lanes per point use that width for every sub-step.
+/// (kHalfBelow, kWideBelow or set_width_thresholds); 16, 32 or 64 lanes per point use that
+/// width for every sub-step.
 void set_group_lanes(int lanes);
+/// The active-point thresholds of the automatic width choice (wide_below <= half_below).
+void set_width_thresholds(int half_below, int wide_below);
 /// W3SRCE's section 3 for every point and the outputs.
 hipError_t k_finish(const State& s, const St4Dev& q, const SrceDev& p, const StepFlags& f,
                     hipStream_t st);

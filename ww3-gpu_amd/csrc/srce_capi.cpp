@@ -1227,4 +1227,14 @@ int ww3gpu_srce_group_lanes(int lanes) {
   return WW_KOKKOS_OK;
 }
 
+int ww3gpu_srce_width_thresholds(int half_below, int wide_below) {
+  if (half_below == 0 && wide_below == 0) {
+    half_below = ww3gpu::srce::kHalfBelow;
+    wide_below = ww3gpu::srce::kWideBelow;
+  }
+  if (half_below < 0 || wide_below < 0 || wide_below > half_below) return WW_KOKKOS_ERR_BAD_SHAPE;
+  ww3gpu::srce::set_width_thresholds(half_below, wide_below);
+  return WW_KOKKOS_OK;
+}
+
 }  // extern "C"

@@ -1987,6 +1987,7 @@ hipError_t launch_step(const State& s, const St4Dev& q, const SrceDev& p, const 
 
 namespace {
 std::atomic<int> g_group_lanes{0};  // set_group_lanes
+std::atomic<int> g_half_below{kHalfBelow}, g_wide_below{kWideBelow};  // set_width_thresholds
 }  // namespace
 
 /// Points that one round of 16-lane-row workgroups holds on the current device: the CUs times
@@ -2014,20 +2015,27 @@ hipError_t k_step(const State& s, const St4Dev& q, const SrceDev& p, const StepF
   if (lanes == 16) return launch_step<16>(s, q, p, f, iter, 0, n, st);
   if (lanes == 32) return launch_step<32>(s, q, p, f, iter, 0, n, st);
   if (lanes == 64) return launch_step<64>(s, q, p, f, iter, 0, n, st);
-  if (n > kHalfBelow) {
+  const int half_below = g_half_below.load(std::memory_order_relaxed);
+  const int wide_below = g_wide_below.load(std::memory_order_relaxed);
+  if (n > half_below) {
     const int round16 = rows_round(q);
-    if (round16 > 0 && n > round16 && n - round16 <= kHalfBelow) {
+    if (round16 > 0 && n > round16 && n - round16 <= half_below) {
       const hipError_t e = launch_step<16>(s, q, p, f, iter, 0, round16, st);
       if (e != hipSuccess) return e;
       return launch_step<32>(s, q, p, f, iter, round16, n, st);
     }
     return launch_step<16>(s, q, p, f, iter, 0, n, st);
   }
-  if (n > kWideBelow) return launch_step<32>(s, q, p, f, iter, 0, n, st);
+  if (n > wide_below) return launch_step<32>(s, q, p, f, iter, 0, n, st);
   return launch_step<64>(s, q, p, f, iter, 0, n, st);
 }
 
 void set_group_lanes(int lanes) { g_group_lanes.store(lanes, std::memory_order_relaxed); }
+
+void set_width_thresholds(int half_below, int wide_below) {
+  g_half_below.store(half_below, std::memory_order_relaxed);
+  g_wide_below.store(wide_below, std::memory_order_relaxed);
+}
 
 hipError_t k_routine(const St4Call& io, const St4Dev& q, hipStream_t st) {
   if (io.npts <= 0) return hipSuccess;

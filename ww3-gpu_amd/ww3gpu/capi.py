@@ -76,6 +76,7 @@ SIGNATURES = {
     "ww3gpu_srce_point_substeps": (c_long, [c_int]),
     "ww3gpu_srce_timing": (c_int, [c_int, c_int]),
     "ww3gpu_srce_group_lanes": (c_int, [c_int]),
+    "ww3gpu_srce_width_thresholds": (c_int, [c_int, c_int]),
     "ww3gpu_srce_timers": (c_int, [c_int, c_void_p, c_void_p, c_int]),
     "ww3gpu_insin4_tables": (c_int, [c_void_p, c_int, c_void_p, c_int] + [c_void_p] * 7),
     "ww_sds4": (None, [c_int, c_int] + [c_void_p] * 8 + [c_int, c_int] + [c_void_p] * 4),
