@@ -122,3 +122,20 @@ def test_thread_count_does_not_change_bits():
     s1, d1 = O.w3snl1(cfg, t, sig, a, cg, kd, nthreads=1)
     s4, d4 = O.w3snl1(cfg, t, sig, a, cg, kd, nthreads=4)
     assert np.array_equal(bits(s1), bits(s4)) and np.array_equal(bits(d1), bits(d4))
+
+
+def test_fp64_evaluation_tracks_the_float32_oracle():
+    """The float64 restatement of W3SNL1 (tests/dia_fp64.py) against the float32 oracle on
+    the 1000-point deep-water case and on log-uniform depths: they differ only by float32
+    rounding (relative L2 below 1e-5), so the float64 evaluation follows the same routine
+    and the float32 path's error against it is rounding."""
+    import dia_fp64
+    from ww3gpu import synth
+    for nth, kw in ((24, {}), (36, {"log_depth": (10.0, 4000.0)})):
+        cfg, sig, a, cg, kd = synth.dia_workload(300, 32, nth, seed=17, **kw)
+        t = O.insnl1(32, nth, cfg["xfr"], cfg["dth"], cfg["lam"], sig)
+        s32, _ = O.w3snl1(cfg, t, sig, a, cg, kd)
+        s64 = dia_fp64.w3snl1_fp64(cfg, t, sig, a, cg, kd)
+        err = dia_fp64.error_vs_fp64(s32, s64)
+        assert err["rel_l2_global"] < 1e-5, err
+        assert err["rel_l2_point_max"] < 1e-4, err

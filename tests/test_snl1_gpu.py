@@ -160,3 +160,22 @@ def test_symmetries_at_full_size(gpu_runtime):
     assert ms.mean() > 0.5 and md.mean() > 0.5
     assert_bits(s4[ms], (s * np.float32(8.0))[ms], "scaling S")
     assert_bits(d4[md], (d * np.float32(4.0))[md], "scaling D")
+
+
+def test_dia_error_against_fp64_evaluation(gpu_runtime):
+    """The device DIA's S_nl (bit-identical to the float32 oracle) against a float64
+    evaluation of the same routine (tests/dia_fp64.py) on the source-term step's shape
+    (NK=32, NTH=36, log-uniform depths): float32 rounding only, relative L2 below 2e-6."""
+    import dia_fp64
+    torch = gpu_runtime
+    cfg, sig, a, cg, kd = synth.dia_workload(2000, 32, 36, seed=2024, log_depth=(10.0, 4000.0))
+    op = Snl1(23, cfg, sig)
+    A, CG, KD = (torch.from_numpy(x).cuda() for x in (a, cg, kd))
+    S = torch.empty_like(A)
+    D = torch.empty_like(A)
+    op.device(A, CG, KD, S, D)
+    torch.cuda.synchronize()
+    t = O.insnl1(32, 36, cfg["xfr"], cfg["dth"], cfg["lam"], sig)
+    err = dia_fp64.error_vs_fp64(S.cpu().numpy(), dia_fp64.w3snl1_fp64(cfg, t, sig, a, cg, kd))
+    print("DIA error vs float64:", err)
+    assert err["rel_l2_global"] < 2e-6 and err["rel_l2_point_max"] < 1e-5, err
