@@ -560,7 +560,7 @@ int ww3gpu_snlsur_dev(int igrid, int npts, const float* a, const float* cg, cons
 /* Test entries: loss per point and the fp32 parameter gradients for B host feature rows
  * x (B, kin) and normalised targets y (B, nout); the raw network output o (M, nout) for
  * M rows x (M, kin); C = A B^T for bf16 A (M, K), B (N, K) on device (M, N multiples of 128,
- * K of 64). */
+ * K of 64, row strides lda, ldb >= K and multiples of 8, ldc >= N; BAD_SHAPE otherwise). */
 int ww3gpu_snlsur_grads(int igrid, int B, const float* x, const float* y, float* loss, float* grad);
 int ww3gpu_snlsur_forward(int igrid, int M, const float* x, float* o);
 int ww3gpu_sur_gemm_dev(int M, int N, int K, const void* a, int lda, const void* b, int ldb, float* c,

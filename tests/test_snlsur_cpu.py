@@ -108,3 +108,15 @@ def test_surrogate_entries_before_init_report_not_initialised():
     assert L.ww3gpu_snlsur_nparams(977, None) < 0
     assert L.ww3gpu_snlsur_dev(977, 1, None, None, None, None, None, None, None) == capi.Status.NOT_INITIALISED
     assert L.ww3gpu_snlsur_last_error() == capi.Status.NOT_INITIALISED
+
+
+def test_gemm_entry_rejects_bad_shapes():
+    """Shapes the GEMM tiles do not cover, short or unaligned row strides and null pointers
+    are BAD_SHAPE before anything is launched."""
+    L = capi.lib()
+    dummy = 0x1000  # never dereferenced: every call below fails validation first
+    bad = [(100, 128, 64, 64, 64, 128), (128, 100, 64, 64, 64, 100), (128, 128, 60, 60, 60, 128),
+           (128, 128, 64, 32, 64, 128), (128, 128, 64, 64, 68, 128), (128, 128, 64, 64, 64, 64)]
+    for m, n, k, lda, ldb, ldc in bad:
+        assert L.ww3gpu_sur_gemm_dev(m, n, k, dummy, lda, dummy, ldb, dummy, ldc, None) == capi.Status.BAD_SHAPE
+    assert L.ww3gpu_sur_gemm_dev(128, 128, 64, None, 64, dummy, 64, dummy, 128, None) == capi.Status.BAD_SHAPE

@@ -42,8 +42,7 @@ struct Net {
   // work buffers
   DevBuf x, xt, y, z[3], h[3], ht[3], o, go, got, dz[2], dzt[2], loss, scale, shift, flag, list, nlist;
   DevBuf fa, fcg, fkd, fs, fd;
-  DevBuf data_a, data_cg, data_kd, data_s, data_d;  // a training set
-  int data_n = 0;
+  DevBuf data_a, data_cg, data_kd, data_s, data_d;  // a training set (released after the fit)
   bool ready = false;
 };
 
@@ -469,7 +468,6 @@ int ww3gpu_snlsur_fit(int igrid, int nsamples, int batch, int epochs, float lr, 
   // Targets from the exact DIA of this library.
   if (ww3gpu_snl1_dev(igrid, nsamples, A, CG, KD, S, n->data_d.as<float>(), st) != WW_KOKKOS_OK)
     return fail(WW_KOKKOS_ERR_KERNEL, "ww3gpu_snlsur_fit: DIA targets");
-  n->data_n = nsamples;
   const int nb = nsamples / batch;
   const long total = static_cast<long>(nb) * epochs;
   std::mt19937_64 rng(seed ^ 0x5bd1e995ull);
@@ -567,6 +565,8 @@ int ww3gpu_snlsur_last_error(void) { return g_last.load(std::memory_order_relaxe
 
 int ww3gpu_sur_gemm_dev(int M, int N, int K, const void* a, int lda, const void* b, int ldb, float* c, int ldc,
                         void* stream) {
+  if (a == nullptr || b == nullptr || c == nullptr || lda < K || ldb < K || ldc < N || lda % 8 || ldb % 8)
+    return WW_KOKKOS_ERR_BAD_SHAPE;
   Epi e{};
   e.mode = kPlain;
   e.of = c;
