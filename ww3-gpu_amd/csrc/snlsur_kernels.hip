@@ -205,6 +205,13 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(int K, int ntn, int T, con
 constexpr int kMaxSpec = 2048;  // NSPEC limit of the normalisation kernels
 constexpr int kMaxK = 64;       // NK limit
 
+// i / nth for 0 <= i < 2048 and nth <= 64 without an integer division: (i + 1/2) / nth sits
+// at least 1/(2 nth) >= 2^-7 from an integer, and the float product errs by at most
+// 2048 * 2^-23 = 2^-12, so truncation gives the exact quotient.
+__device__ __forceinline__ int div_nth(int i, float inv_nth) {
+  return static_cast<int>((static_cast<float>(i) + 0.5f) * inv_nth);
+}
+
 __device__ __forceinline__ float conx_of(const Norm& nm, const float* cgr, int k) {
   return nm.tpiinv / nm.sig[k] * cgr[k];
 }
@@ -220,6 +227,7 @@ __global__ __launch_bounds__(256) void prep_kernel(Norm nm, int npts, const floa
   const int p = blockIdx.x * 4 + wv;
   const bool live = p < npts;
   const int nspec = nm.nspec, nth = nm.nth, kin = nm.kin, nout = nm.nout;
+  const float inv_nth = 1.0f / static_cast<float>(nth);
   uint16_t* xr = x + static_cast<size_t>(p) * kin;
   float* yr = y != nullptr ? y + static_cast<size_t>(p) * nout : nullptr;
   const float* ar = a + static_cast<size_t>(p) * nspec;
@@ -233,7 +241,7 @@ __global__ __launch_bounds__(256) void prep_kernel(Norm nm, int npts, const floa
   if (live) {
 #pragma unroll 6
     for (int i = lane; i < nspec; i += 64) {
-      const float u = ar[i] * icx[wv][i / nth];
+      const float u = ar[i] * icx[wv][div_nth(i, inv_nth)];
       ue[i] = u;
       if (!(u >= 0.0f) || isinf(u)) bad = true;
       if (u > best) {
@@ -260,7 +268,7 @@ __global__ __launch_bounds__(256) void prep_kernel(Norm nm, int npts, const floa
   float q = 0.0f;
 #pragma unroll 6
   for (int i = lane; i < nspec; i += 64) {
-    const int k = i / nth, t = i - k * nth;
+    const int k = div_nth(i, inv_nth), t = i - k * nth;
     int ts = t + jp;
     if (ts >= nth) ts -= nth;
     const float u = bad ? 0.0f : ue[k * nth + ts] * inv_m;
@@ -289,7 +297,7 @@ __global__ __launch_bounds__(256) void prep_kernel(Norm nm, int npts, const floa
     for (int i = lane; i < nout; i += 64) {
       float v = 0.0f;
       if (i < nspec && !bad) {
-        const int k = i / nth, t = i - k * nth;
+        const int k = div_nth(i, inv_nth), t = i - k * nth;
         int ts = t + jp;
         if (ts >= nth) ts -= nth;
         v = s_ref[static_cast<size_t>(p) * nspec + k * nth + ts] * icx[wv][k] * inv_sc;
@@ -318,6 +326,7 @@ __global__ __launch_bounds__(256) void post_kernel(Norm nm, int npts, const uint
   __syncthreads();
   if (!live) return;
   const int nspec = nm.nspec, nth = nm.nth;
+  const float inv_nth = 1.0f / static_cast<float>(nth);
   const int jp = shift[p];
   const uint16_t* orow = o + static_cast<size_t>(p) * nm.nout;
   float* sr = s + static_cast<size_t>(p) * nspec;
@@ -325,7 +334,7 @@ __global__ __launch_bounds__(256) void post_kernel(Norm nm, int npts, const uint
   bool nonfinite = false;
 #pragma unroll 6
   for (int i = lane; i < nspec; i += 64) {
-    const int k = i / nth, t = i - k * nth;
+    const int k = div_nth(i, inv_nth), t = i - k * nth;
     int tc = t - jp;
     if (tc < 0) tc += nth;
     const float v = __uint_as_float(static_cast<uint32_t>(orow[k * nth + tc]) << 16) * cx[wv][k];
